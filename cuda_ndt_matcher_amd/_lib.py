@@ -1,0 +1,148 @@
+"""ctypes binding of libndt_hip.so (the C ABI in include/ndt_hip.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+entry point raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libndt_hip.so")
+
+NDT_OK = 0
+ERROR_NAMES = {
+    -1: "INVALID_ARGUMENT",
+    -2: "EMPTY_INPUT",
+    -3: "NO_TARGET",
+    -4: "NO_VOXELS",
+    -5: "HIP",
+    -6: "UNSUPPORTED",
+    -7: "CAPACITY",
+}
+
+
+class NdtConfig(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_float),
+        ("max_iterations", C.c_uint32),
+        ("trans_epsilon", C.c_double),
+        ("step_size", C.c_double),
+        ("outlier_ratio", C.c_double),
+        ("regularization", C.c_double),
+        ("use_line_search", C.c_int32),
+        ("min_points_per_voxel", C.c_uint32),
+        ("regularization_enabled", C.c_int32),
+        ("regularization_scale_factor", C.c_double),
+        ("distance_metric", C.c_int32),
+        ("search_mode", C.c_int32),
+        ("precision", C.c_int32),
+        ("workgroups_per_alignment", C.c_int32),
+    ]
+
+
+class NdtResult(C.Structure):
+    _fields_ = [
+        ("pose", C.c_double * 6),
+        ("status", C.c_int32),
+        ("converged", C.c_int32),
+        ("score", C.c_double),
+        ("transform_probability", C.c_double),
+        ("nvtl", C.c_double),
+        ("iterations", C.c_uint32),
+        ("num_correspondences", C.c_uint32),
+        ("oscillation_count", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("hessian", C.c_double * 36),
+    ]
+
+
+class NdtTiming(C.Structure):
+    _fields_ = [
+        ("build_ms", C.c_float),
+        ("sort_ms", C.c_float),
+        ("align_ms", C.c_float),
+        ("score_ms", C.c_float),
+        ("align_workgroups", C.c_uint32),
+        ("align_slots", C.c_uint32),
+    ]
+
+
+# Every symbol declared in include/ndt_hip.h.
+EXPORTS = [
+    "ndt_hip_config_default", "ndt_hip_create", "ndt_hip_destroy", "ndt_hip_last_error",
+    "ndt_hip_set_target", "ndt_hip_set_target_device", "ndt_hip_has_target", "ndt_hip_num_voxels",
+    "ndt_hip_get_voxels", "ndt_hip_get_build_debug", "ndt_hip_lookup", "ndt_hip_derivatives",
+    "ndt_hip_align", "ndt_hip_align_device", "ndt_hip_align_batch", "ndt_hip_align_batch_device",
+    "ndt_hip_align_scans", "ndt_hip_evaluate_nvtl", "ndt_hip_evaluate_transform_probability",
+    "ndt_hip_per_point_scores", "ndt_hip_evaluate_nvtl_batch", "ndt_hip_set_regularization_pose",
+    "ndt_hip_clear_regularization_pose", "ndt_hip_laplace_covariance_xy", "ndt_hip_laplace_covariance_6x6",
+    "ndt_hip_get_timing",
+]
+
+_lib = None
+
+
+def load():
+    """Loads libndt_hip.so (raises OSError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C cuda_ndt_matcher_amd/csrc")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+    sz = C.c_size_t
+    rp = C.POINTER(NdtResult)
+    lib.ndt_hip_config_default.argtypes = [C.POINTER(NdtConfig)]
+    lib.ndt_hip_config_default.restype = None
+    lib.ndt_hip_create.argtypes = [C.POINTER(NdtConfig), C.POINTER(vp)]
+    lib.ndt_hip_destroy.argtypes = [vp]
+    lib.ndt_hip_destroy.restype = None
+    lib.ndt_hip_last_error.argtypes = [vp]
+    lib.ndt_hip_last_error.restype = C.c_char_p
+    lib.ndt_hip_set_target.argtypes = [vp, f32p, sz]
+    lib.ndt_hip_set_target_device.argtypes = [vp, vp, sz]
+    lib.ndt_hip_has_target.argtypes = [vp]
+    lib.ndt_hip_num_voxels.argtypes = [vp, C.POINTER(sz)]
+    lib.ndt_hip_get_voxels.argtypes = [vp, i32p, f32p, f32p, f32p, f32p, u32p]
+    lib.ndt_hip_get_build_debug.argtypes = [vp, i32p, u32p, u64p]
+    lib.ndt_hip_lookup.argtypes = [vp, C.c_int32, f32p, sz, C.c_uint32, u32p, i32p]
+    lib.ndt_hip_derivatives.argtypes = [vp, f32p, sz, f64p, C.c_int32, C.POINTER(C.c_double), f64p, f64p,
+                                        C.POINTER(C.c_uint32)]
+    lib.ndt_hip_align.argtypes = [vp, f32p, sz, f64p, rp]
+    lib.ndt_hip_align_device.argtypes = [vp, vp, sz, f64p, rp]
+    lib.ndt_hip_align_batch.argtypes = [vp, f32p, sz, f64p, sz, rp]
+    lib.ndt_hip_align_batch_device.argtypes = [vp, vp, sz, f64p, sz, rp]
+    lib.ndt_hip_align_scans.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(sz), f64p, sz, rp]
+    for name in ("ndt_hip_evaluate_nvtl", "ndt_hip_evaluate_transform_probability"):
+        getattr(lib, name).argtypes = [vp, f32p, sz, f64p, C.POINTER(C.c_double)]
+    lib.ndt_hip_per_point_scores.argtypes = [vp, f32p, sz, f64p, f64p]
+    lib.ndt_hip_evaluate_nvtl_batch.argtypes = [vp, f32p, sz, f64p, sz, f64p]
+    lib.ndt_hip_set_regularization_pose.argtypes = [vp, f64p]
+    lib.ndt_hip_clear_regularization_pose.argtypes = [vp]
+    lib.ndt_hip_laplace_covariance_xy.argtypes = [f64p, f64p]
+    lib.ndt_hip_laplace_covariance_6x6.argtypes = [f64p, f64p]
+    lib.ndt_hip_get_timing.argtypes = [vp, C.POINTER(NdtTiming)]
+    _lib = lib
+    return lib
+
+
+def missing_exports() -> list:
+    """Names declared in include/ndt_hip.h that the library does not export."""
+    lib = load()
+    out = []
+    for name in EXPORTS:
+        try:
+            getattr(lib, name)
+        except AttributeError:
+            out.append(name)
+    return out

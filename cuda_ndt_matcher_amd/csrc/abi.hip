@@ -1,0 +1,612 @@
+// C ABI of libndt_hip.so (declared in include/ndt_hip.h).
+//
+// Host orchestration only: argument checks with the reference's error
+// classes, grow-only device buffers owned by the handle, one stream per
+// handle, and the launch shapes.  All arithmetic runs in the HIP kernels;
+// there is no CPU fallback.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ndt_hip.h"
+#include "ndt_device.hpp"
+#include "ndt_internal.hpp"
+
+using ndt::DevBuf;
+
+struct ndt_handle {
+  ndt_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ndt::DeviceGrid grid;
+  std::string err;
+  DevBuf scan, jobs, results, ws, poses, partials, out, lookup_q, lookup_c, lookup_i;
+  std::vector<const float*> scan_ptrs;
+  ndt_result* host_results = nullptr;
+  size_t host_results_cap = 0;
+  bool has_reg_pose = false;
+  double reg_t[3] = {0, 0, 0};
+  hipEvent_t ev0{}, ev1{};
+  ndt_timing timing{};
+  int coop_capacity = 0;
+  double d1 = 0, d2 = 0;
+  std::vector<int32_t> slot_to_index;  // voxel slot -> exported index (-1 if dropped)
+};
+
+namespace {
+
+int set_err(ndt_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+int hip_err(ndt_handle* h, hipError_t e, const char* what) {
+  return set_err(h, NDT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define NDT_HIP(call, what)                      \
+  do {                                           \
+    hipError_t e_ = (call);                      \
+    if (e_ != hipSuccess) return hip_err(h, e_, what); \
+  } while (0)
+
+void gauss_params(double res, double outlier, double* d1, double* d2) {
+  // GaussianParams::new (derivatives/types.rs:61-75)
+  const double c1 = 10.0 * (1.0 - outlier);
+  const double c2 = outlier / (res * res * res);
+  const double d3 = -std::log(c2);
+  *d1 = -std::log(c1 + c2) - d3;
+  const double nom = -std::log(c1 * std::exp(-0.5) + c2) - d3;
+  *d2 = -2.0 * std::log(nom / *d1);
+}
+
+ndt::NewtonConfig newton_config(const ndt_handle* h, int wg) {
+  ndt::NewtonConfig c{};
+  c.d1 = h->d1;
+  c.d2 = h->d2;
+  c.trans_eps = h->cfg.trans_epsilon;
+  c.step_size = h->cfg.step_size;
+  c.reg_scale = h->cfg.regularization_scale_factor;
+  for (int i = 0; i < 3; ++i) c.reg_t[i] = h->reg_t[i];
+  c.max_iter = h->cfg.max_iterations;
+  c.search_mode = h->cfg.search_mode;
+  c.metric = h->cfg.distance_metric;
+  c.use_ls = h->cfg.use_line_search;
+  c.reg_active = (h->cfg.regularization_enabled && h->has_reg_pose) ? 1 : 0;
+  c.precision = h->cfg.precision;
+  c.wg_per_slot = wg;
+  return c;
+}
+
+int check_ready(ndt_handle* h) {
+  if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  if (!h->grid.ready) return set_err(h, NDT_ERR_NO_TARGET, "No target set. Call set_target() first.");
+  NDT_HIP(hipSetDevice(h->device), "hipSetDevice");
+  return NDT_OK;
+}
+
+int upload_points(ndt_handle* h, DevBuf& buf, const float* xyz, size_t n, const float** dptr) {
+  NDT_HIP(buf.reserve(n * 3 * sizeof(float)), "alloc points");
+  NDT_HIP(hipMemcpyAsync(buf.ptr, xyz, n * 3 * sizeof(float), hipMemcpyHostToDevice, h->stream), "copy points");
+  *dptr = reinterpret_cast<const float*>(buf.ptr);
+  return NDT_OK;
+}
+
+int ensure_host_results(ndt_handle* h, size_t m) {
+  if (m <= h->host_results_cap) return NDT_OK;
+  if (h->host_results) (void)hipHostFree(h->host_results);
+  h->host_results = nullptr;
+  NDT_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->host_results), m * sizeof(ndt_result), hipHostMallocDefault),
+          "pinned alloc");
+  h->host_results_cap = m;
+  return NDT_OK;
+}
+
+// Workgroups per alignment: about two points per thread, capped so that all
+// slots' workgroups are co-resident (the slot barrier needs that).
+int choose_wg(const ndt_handle* h, size_t n_max, size_t m) {
+  const int cap = std::max(1, h->coop_capacity);
+  int per_slot_cap = (int)std::max<size_t>(1, (size_t)cap / std::max<size_t>(1, m));
+  if (h->cfg.workgroups_per_alignment > 0) return std::min(h->cfg.workgroups_per_alignment, per_slot_cap);
+  const size_t want = (n_max + 511) / 512;
+  return (int)std::max<size_t>(1, std::min<size_t>(want, (size_t)per_slot_cap));
+}
+
+// Runs m alignments: scan i = (d_pts[i], n[i]) from poses[6*i].
+int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, const double* poses, size_t m,
+                   ndt_result* out) {
+  size_t n_max = 0;
+  for (size_t i = 0; i < m; ++i) {
+    if (n[i] == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+    if (n[i] >= (1ull << 31)) return set_err(h, NDT_ERR_CAPACITY, "scan exceeds 2^31 points");
+    n_max = std::max(n_max, n[i]);
+  }
+  const int W = choose_wg(h, n_max, m);
+  if ((size_t)W * m > (size_t)std::max(1, h->coop_capacity) && W > 1)
+    return set_err(h, NDT_ERR_CAPACITY, "too many cooperating workgroups for one launch");
+  std::vector<ndt::SlotJob> jobs(m);
+  for (size_t i = 0; i < m; ++i) {
+    jobs[i].pts = d_pts[i];
+    jobs[i].n = (uint32_t)n[i];
+    jobs[i].pad = 0;
+    for (int k = 0; k < 6; ++k) jobs[i].pose[k] = poses[6 * i + k];
+  }
+  NDT_HIP(h->jobs.reserve(m * sizeof(ndt::SlotJob)), "alloc jobs");
+  NDT_HIP(h->results.reserve(m * sizeof(ndt_result)), "alloc results");
+  NDT_HIP(h->ws.reserve(ndt::newton_workspace_bytes((int)m, W)), "alloc workspace");
+  int rc = ensure_host_results(h, m);
+  if (rc) return rc;
+  NDT_HIP(hipMemcpyAsync(h->jobs.ptr, jobs.data(), m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
+          "copy jobs");
+  const ndt::NewtonConfig cfg = newton_config(h, W);
+  NDT_HIP(hipEventRecord(h->ev0, h->stream), "event");
+  NDT_HIP(ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr), (int)m,
+                             reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream, W > 1),
+          "newton launch");
+  NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
+  NDT_HIP(hipMemcpyAsync(h->host_results, h->results.ptr, m * sizeof(ndt_result), hipMemcpyDeviceToHost, h->stream),
+          "copy results");
+  NDT_HIP(hipStreamSynchronize(h->stream), "newton sync");
+  float ms = 0.f;
+  NDT_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time");
+  h->timing.align_ms = ms;
+  h->timing.align_workgroups = (uint32_t)(W * m);
+  h->timing.align_slots = (uint32_t)m;
+  std::memcpy(out, h->host_results, m * sizeof(ndt_result));
+  return NDT_OK;
+}
+
+int run_score(ndt_handle* h, int kind, const float* d_pts, size_t n, const double* poses, size_t m, double* out) {
+  NDT_HIP(h->poses.reserve(m * 6 * sizeof(double)), "alloc poses");
+  NDT_HIP(hipMemcpyAsync(h->poses.ptr, poses, m * 6 * sizeof(double), hipMemcpyHostToDevice, h->stream), "copy poses");
+  const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 512));
+  NDT_HIP(h->partials.reserve((m * blocks * 32 + 2 * m + 64) * sizeof(double)), "alloc partials");
+  const size_t out_n = (kind == ndt::kScorePerPoint) ? n : m;
+  NDT_HIP(h->out.reserve(out_n * sizeof(double)), "alloc out");
+  NDT_HIP(hipEventRecord(h->ev0, h->stream), "event");
+  NDT_HIP(ndt::launch_score(h->grid.view, h->d1, h->d2, kind, d_pts, (uint32_t)n,
+                            reinterpret_cast<const double*>(h->poses.ptr), (int)m,
+                            reinterpret_cast<double*>(h->partials.ptr), blocks,
+                            reinterpret_cast<double*>(h->out.ptr), h->stream),
+          "score launch");
+  NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
+  NDT_HIP(hipMemcpyAsync(out, h->out.ptr, out_n * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy scores");
+  NDT_HIP(hipStreamSynchronize(h->stream), "score sync");
+  float ms = 0.f;
+  NDT_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time");
+  h->timing.score_ms = ms;
+  return NDT_OK;
+}
+
+int finish_target(ndt_handle* h, size_t n) {
+  std::string err;
+  const int rc = ndt::build_grid(h->grid, n, h->cfg.resolution, h->cfg.min_points_per_voxel, h->stream, &err);
+  h->timing.build_ms = h->grid.build_ms;
+  h->timing.sort_ms = h->grid.sort_ms;
+  if (rc != 0) {
+    h->grid.ready = false;
+    return set_err(h, rc, err);
+  }
+  // slot -> exported index map
+  std::vector<uint32_t> valid(h->grid.segments);
+  NDT_HIP(hipMemcpy(valid.data(), h->grid.valid.ptr, valid.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+          "copy valid flags");
+  h->slot_to_index.assign(h->grid.segments, -1);
+  int32_t k = 0;
+  for (uint32_t s = 0; s < h->grid.segments; ++s)
+    if (valid[s]) h->slot_to_index[s] = k++;
+  h->grid.ready = true;
+  return NDT_OK;
+}
+
+bool valid_config(const ndt_config* c, std::string* why) {
+  if (!(c->resolution > 0.0f) || !std::isfinite(c->resolution)) return *why = "resolution must be positive", false;
+  if (c->max_iterations > 256) return *why = "max_iterations above 256 is not supported", false;
+  if (c->search_mode < 0 || c->search_mode > 3) return *why = "unknown search_mode", false;
+  if (c->distance_metric < 0 || c->distance_metric > 1) return *why = "unknown distance_metric", false;
+  if (c->precision < 0 || c->precision > 1) return *why = "unknown precision", false;
+  if (!(c->outlier_ratio > 0.0 && c->outlier_ratio < 1.0)) return *why = "outlier_ratio must be in (0,1)", false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ndt_hip_config_default(ndt_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->resolution = 2.0f;
+  c->max_iterations = 30;
+  c->trans_epsilon = 0.01;
+  c->step_size = 0.1;
+  c->outlier_ratio = 0.55;
+  c->regularization = 0.001;
+  c->use_line_search = 0;
+  c->min_points_per_voxel = 6;
+  c->regularization_enabled = 0;
+  c->regularization_scale_factor = 0.01;
+  c->distance_metric = NDT_METRIC_POINT_TO_DISTRIBUTION;
+  c->search_mode = NDT_SEARCH_KDTREE;
+  c->precision = NDT_PRECISION_FP64;
+  c->workgroups_per_alignment = 0;
+}
+
+int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
+  if (!cfg || !out) return NDT_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::string why;
+  if (!valid_config(cfg, &why)) return NDT_ERR_INVALID_ARGUMENT;
+  if (cfg->precision != NDT_PRECISION_FP64) return NDT_ERR_UNSUPPORTED;
+  auto* h = new ndt_handle();
+  h->cfg = *cfg;
+  gauss_params((double)cfg->resolution, cfg->outlier_ratio, &h->d1, &h->d2);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = hipEventCreate(&h->grid.ev_build0);
+  if (e == hipSuccess) e = hipEventCreate(&h->grid.ev_build1);
+  if (e == hipSuccess) e = hipEventCreate(&h->grid.ev_sort0);
+  if (e == hipSuccess) e = hipEventCreate(&h->grid.ev_sort1);
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&h->grid.host_scalars), 64 * sizeof(uint32_t), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    ndt_hip_destroy(h);
+    return NDT_ERR_HIP;
+  }
+  h->coop_capacity = ndt::newton_max_coresident_workgroups();
+  *out = h;
+  return NDT_OK;
+}
+
+void ndt_hip_destroy(ndt_handle* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  ndt::DeviceGrid& G = h->grid;
+  for (DevBuf* b : {&G.owned_points, &G.sort_keys_a, &G.sort_keys_b, &G.sort_vals_a, &G.sort_vals_b, &G.hist,
+                    &G.status, &G.tile_heads, &G.seg_start, &G.scalars, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
+                    &G.valid, &G.table, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
+                    &h->lookup_q, &h->lookup_c, &h->lookup_i})
+    b->release();
+  if (G.host_scalars) (void)hipHostFree(G.host_scalars);
+  if (h->host_results) (void)hipHostFree(h->host_results);
+  for (hipEvent_t ev : {h->ev0, h->ev1, G.ev_build0, G.ev_build1, G.ev_sort0, G.ev_sort1})
+    if (ev) (void)hipEventDestroy(ev);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+const char* ndt_hip_last_error(const ndt_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int ndt_hip_set_target(ndt_handle* h, const float* xyz, size_t n) {
+  if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  if (n == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Target point cloud is empty");
+  if (!xyz) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null points");
+  NDT_HIP(hipSetDevice(h->device), "hipSetDevice");
+  const float* d = nullptr;
+  int rc = upload_points(h, h->grid.owned_points, xyz, n, &d);
+  if (rc) return rc;
+  h->grid.points = d;
+  return finish_target(h, n);
+}
+
+int ndt_hip_set_target_device(ndt_handle* h, const float* d_xyz, size_t n) {
+  if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  if (n == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Target point cloud is empty");
+  if (!d_xyz) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null points");
+  NDT_HIP(hipSetDevice(h->device), "hipSetDevice");
+  h->grid.points = d_xyz;
+  return finish_target(h, n);
+}
+
+int ndt_hip_has_target(const ndt_handle* h) { return (h && h->grid.ready) ? 1 : 0; }
+
+int ndt_hip_num_voxels(const ndt_handle* h, size_t* out) {
+  if (!h || !out) return NDT_ERR_INVALID_ARGUMENT;
+  *out = h->grid.ready ? h->grid.num_valid : 0;
+  return NDT_OK;
+}
+
+int ndt_hip_get_voxels(ndt_handle* h, int32_t* coords, float* means, float* covs, float* invs, float* axes,
+                       uint32_t* counts) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  const uint32_t S = h->grid.segments;
+  std::vector<float> vox(12 * (size_t)S), cov(9 * (size_t)S), ax(3 * (size_t)S);
+  std::vector<uint32_t> cnt(S);
+  std::vector<int32_t> crd(3 * (size_t)S);
+  NDT_HIP(hipStreamSynchronize(h->stream), "sync");
+  NDT_HIP(hipMemcpy(vox.data(), h->grid.vox.ptr, vox.size() * 4, hipMemcpyDeviceToHost), "copy voxels");
+  NDT_HIP(hipMemcpy(cov.data(), h->grid.cov.ptr, cov.size() * 4, hipMemcpyDeviceToHost), "copy voxels");
+  NDT_HIP(hipMemcpy(ax.data(), h->grid.axis.ptr, ax.size() * 4, hipMemcpyDeviceToHost), "copy voxels");
+  NDT_HIP(hipMemcpy(cnt.data(), h->grid.count.ptr, cnt.size() * 4, hipMemcpyDeviceToHost), "copy voxels");
+  NDT_HIP(hipMemcpy(crd.data(), h->grid.coord.ptr, crd.size() * 4, hipMemcpyDeviceToHost), "copy voxels");
+  for (uint32_t s = 0; s < S; ++s) {
+    const int32_t k = h->slot_to_index[s];
+    if (k < 0) continue;
+    const float* v = &vox[12 * (size_t)s];
+    if (coords) std::memcpy(coords + 3 * (size_t)k, &crd[3 * (size_t)s], 12);
+    if (means) std::memcpy(means + 3 * (size_t)k, v, 12);
+    if (invs) std::memcpy(invs + 9 * (size_t)k, v + 3, 36);
+    if (covs) std::memcpy(covs + 9 * (size_t)k, &cov[9 * (size_t)s], 36);
+    if (axes) std::memcpy(axes + 3 * (size_t)k, &ax[3 * (size_t)s], 12);
+    if (counts) counts[k] = cnt[s];
+  }
+  return NDT_OK;
+}
+
+int ndt_hip_get_build_debug(ndt_handle* h, int32_t* point_keys, uint32_t* sorted_index, uint64_t* sorted_keys) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  const size_t n = h->grid.n_points;
+  std::vector<uint64_t> keys(n);
+  std::vector<uint32_t> idx(n);
+  NDT_HIP(hipStreamSynchronize(h->stream), "sync");
+  if (h->grid.key64) {
+    NDT_HIP(hipMemcpy(keys.data(), h->grid.sorted_keys, n * 8, hipMemcpyDeviceToHost), "copy keys");
+  } else {
+    std::vector<uint32_t> k32(n);
+    NDT_HIP(hipMemcpy(k32.data(), h->grid.sorted_keys, n * 4, hipMemcpyDeviceToHost), "copy keys");
+    for (size_t i = 0; i < n; ++i) keys[i] = k32[i];
+  }
+  NDT_HIP(hipMemcpy(idx.data(), h->grid.sorted_vals, n * 4, hipMemcpyDeviceToHost), "copy order");
+  if (sorted_index) std::memcpy(sorted_index, idx.data(), n * 4);
+  if (sorted_keys) std::memcpy(sorted_keys, keys.data(), n * 8);
+  if (point_keys) {
+    const int* bits = h->grid.key_bits_axis;
+    const int nb = std::max(bits[0], std::max(bits[1], bits[2]));
+    for (size_t j = 0; j < n; ++j) {
+      uint32_t r[3] = {0, 0, 0};
+      int pos = 0;
+      for (int i = 0; i < nb; ++i)
+        for (int a = 0; a < 3; ++a)
+          if (i < bits[a]) r[a] |= (uint32_t)((keys[j] >> pos++) & 1ull) << i;
+      const size_t p = idx[j];
+      for (int a = 0; a < 3; ++a) point_keys[3 * p + a] = (int32_t)((int64_t)r[a] + h->grid.key_min[a]);
+    }
+  }
+  return NDT_OK;
+}
+
+int ndt_hip_lookup(ndt_handle* h, int32_t mode, const float* q, size_t n, uint32_t max_k, uint32_t* counts,
+                   int32_t* indices) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (mode < 0 || mode > 4 || !q || !counts || (max_k && !indices))
+    return set_err(h, NDT_ERR_INVALID_ARGUMENT, "bad lookup arguments");
+  if (n == 0) return NDT_OK;
+  const float* dq = nullptr;
+  rc = upload_points(h, h->lookup_q, q, n, &dq);
+  if (rc) return rc;
+  NDT_HIP(h->lookup_c.reserve(n * 4), "alloc");
+  NDT_HIP(h->lookup_i.reserve(n * std::max<uint32_t>(1, max_k) * 4), "alloc");
+  NDT_HIP(ndt::launch_lookup(h->grid.view, mode, dq, (uint32_t)n, max_k, reinterpret_cast<uint32_t*>(h->lookup_c.ptr),
+                             reinterpret_cast<int32_t*>(h->lookup_i.ptr), h->stream),
+          "lookup launch");
+  NDT_HIP(hipMemcpyAsync(counts, h->lookup_c.ptr, n * 4, hipMemcpyDeviceToHost, h->stream), "copy");
+  if (max_k)
+    NDT_HIP(hipMemcpyAsync(indices, h->lookup_i.ptr, n * max_k * 4, hipMemcpyDeviceToHost, h->stream), "copy");
+  NDT_HIP(hipStreamSynchronize(h->stream), "lookup sync");
+  for (size_t i = 0; i < n * max_k; ++i)
+    if (indices[i] >= 0) indices[i] = h->slot_to_index[indices[i]];
+  return NDT_OK;
+}
+
+int ndt_hip_derivatives(ndt_handle* h, const float* xyz, size_t n, const double pose[6], int32_t compute_hessian,
+                        double* score, double grad[6], double hess[36], uint32_t* num_correspondences) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!xyz || !pose || n == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  const float* d = nullptr;
+  rc = upload_points(h, h->scan, xyz, n, &d);
+  if (rc) return rc;
+  const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 1024));
+  NDT_HIP(h->partials.reserve((size_t)blocks * 32 * sizeof(double)), "alloc");
+  NDT_HIP(h->out.reserve(64 * sizeof(double)), "alloc");
+  NDT_HIP(ndt::launch_derivatives(h->grid.view, newton_config(h, 1), d, (uint32_t)n, pose, compute_hessian,
+                                  reinterpret_cast<double*>(h->partials.ptr), blocks,
+                                  reinterpret_cast<double*>(h->out.ptr), h->stream),
+          "derivative launch");
+  double tot[32];
+  NDT_HIP(hipMemcpyAsync(tot, h->out.ptr, 29 * sizeof(double), hipMemcpyDeviceToHost, h->stream), "copy");
+  NDT_HIP(hipStreamSynchronize(h->stream), "sync");
+  if (score) *score = tot[0];
+  if (grad)
+    for (int i = 0; i < 6; ++i) grad[i] = tot[1 + i];
+  if (hess)
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j <= i; ++j) hess[i * 6 + j] = hess[j * 6 + i] = tot[7 + i * (i + 1) / 2 + j];
+  if (num_correspondences) *num_correspondences = (uint32_t)tot[28];
+  return NDT_OK;
+}
+
+int ndt_hip_align(ndt_handle* h, const float* xyz, size_t n, const double pose[6], ndt_result* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!pose || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  if (n == 0 || !xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  const float* d = nullptr;
+  rc = upload_points(h, h->scan, xyz, n, &d);
+  if (rc) return rc;
+  return run_alignments(h, &d, &n, pose, 1, out);
+}
+
+int ndt_hip_align_device(ndt_handle* h, const float* d_xyz, size_t n, const double pose[6], ndt_result* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!pose || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  if (n == 0 || !d_xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  return run_alignments(h, &d_xyz, &n, pose, 1, out);
+}
+
+int ndt_hip_align_batch(ndt_handle* h, const float* xyz, size_t n, const double* poses, size_t m, ndt_result* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (n == 0 || !xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  if (m == 0) return NDT_OK;
+  if (!poses || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  const float* d = nullptr;
+  rc = upload_points(h, h->scan, xyz, n, &d);
+  if (rc) return rc;
+  std::vector<const float*> ptrs(m, d);
+  std::vector<size_t> ns(m, n);
+  return run_alignments(h, ptrs.data(), ns.data(), poses, m, out);
+}
+
+int ndt_hip_align_batch_device(ndt_handle* h, const float* d_xyz, size_t n, const double* poses, size_t m,
+                               ndt_result* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (n == 0 || !d_xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  if (m == 0) return NDT_OK;
+  if (!poses || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  std::vector<const float*> ptrs(m, d_xyz);
+  std::vector<size_t> ns(m, n);
+  return run_alignments(h, ptrs.data(), ns.data(), poses, m, out);
+}
+
+int ndt_hip_align_scans(ndt_handle* h, const float* const* xyz, const size_t* n, const double* poses, size_t m,
+                        ndt_result* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (m == 0) return NDT_OK;
+  if (!xyz || !n || !poses || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  size_t total = 0;
+  for (size_t i = 0; i < m; ++i) {
+    if (n[i] == 0 || !xyz[i]) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+    total += n[i];
+  }
+  NDT_HIP(h->scan.reserve(total * 3 * sizeof(float)), "alloc scans");
+  std::vector<const float*> ptrs(m);
+  size_t off = 0;
+  for (size_t i = 0; i < m; ++i) {
+    float* dst = reinterpret_cast<float*>(h->scan.ptr) + 3 * off;
+    NDT_HIP(hipMemcpyAsync(dst, xyz[i], n[i] * 3 * sizeof(float), hipMemcpyHostToDevice, h->stream), "copy scan");
+    ptrs[i] = dst;
+    off += n[i];
+  }
+  return run_alignments(h, ptrs.data(), n, poses, m, out);
+}
+
+int ndt_hip_evaluate_nvtl(ndt_handle* h, const float* xyz, size_t n, const double pose[6], double* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (n == 0 || !xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  if (!pose || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  const float* d = nullptr;
+  rc = upload_points(h, h->scan, xyz, n, &d);
+  if (rc) return rc;
+  return run_score(h, ndt::kScoreNvtl, d, n, pose, 1, out);
+}
+
+int ndt_hip_evaluate_transform_probability(ndt_handle* h, const float* xyz, size_t n, const double pose[6],
+                                           double* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (n == 0 || !xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  if (!pose || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  const float* d = nullptr;
+  rc = upload_points(h, h->scan, xyz, n, &d);
+  if (rc) return rc;
+  return run_score(h, ndt::kScoreTp, d, n, pose, 1, out);
+}
+
+int ndt_hip_per_point_scores(ndt_handle* h, const float* xyz, size_t n, const double pose[6], double* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (n == 0) return NDT_OK;  // ndt.rs:811-813 returns an empty vector
+  if (!xyz || !pose || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  const float* d = nullptr;
+  rc = upload_points(h, h->scan, xyz, n, &d);
+  if (rc) return rc;
+  return run_score(h, ndt::kScorePerPoint, d, n, pose, 1, out);
+}
+
+int ndt_hip_evaluate_nvtl_batch(ndt_handle* h, const float* xyz, size_t n, const double* poses, size_t m,
+                                double* out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (m == 0) return NDT_OK;
+  if (!poses || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  if (n == 0) {  // ndt.rs:846-848: zeros for an empty cloud
+    for (size_t i = 0; i < m; ++i) out[i] = 0.0;
+    return NDT_OK;
+  }
+  if (!xyz) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null points");
+  const float* d = nullptr;
+  rc = upload_points(h, h->scan, xyz, n, &d);
+  if (rc) return rc;
+  return run_score(h, ndt::kScoreNvtlCell27, d, n, poses, m, out);
+}
+
+int ndt_hip_set_regularization_pose(ndt_handle* h, const double pose[6]) {
+  if (!h || !pose) return NDT_ERR_INVALID_ARGUMENT;
+  h->has_reg_pose = true;
+  for (int i = 0; i < 3; ++i) h->reg_t[i] = pose[i];
+  return NDT_OK;
+}
+
+int ndt_hip_clear_regularization_pose(ndt_handle* h) {
+  if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  h->has_reg_pose = false;
+  return NDT_OK;
+}
+
+int ndt_hip_laplace_covariance_xy(const double hess[36], double cov[4]) {
+  if (!hess || !cov) return NDT_ERR_INVALID_ARGUMENT;
+  const double a = hess[0], b = hess[1], c = hess[6], d = hess[7];
+  const double det = a * d - b * c;
+  if (det == 0.0) {
+    cov[0] = 1.0; cov[1] = 0.0; cov[2] = 0.0; cov[3] = 1.0;
+    return NDT_OK;
+  }
+  cov[0] = -(d / det);
+  cov[1] = b / det;
+  cov[2] = c / det;
+  cov[3] = -(a / det);
+  return NDT_OK;
+}
+
+int ndt_hip_laplace_covariance_6x6(const double hess[36], double cov[36]) {
+  if (!hess || !cov) return NDT_ERR_INVALID_ARGUMENT;
+  // -H^{-1} by Gauss-Jordan with partial pivoting; identity if singular.
+  double a[6][12];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 12; ++j) a[i][j] = j < 6 ? hess[i * 6 + j] : (j - 6 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 6; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+    if (a[p][c] == 0.0) {
+      for (int i = 0; i < 36; ++i) cov[i] = (i % 7 == 0) ? 1.0 : 0.0;
+      return NDT_OK;
+    }
+    if (p != c)
+      for (int j = 0; j < 12; ++j) std::swap(a[p][j], a[c][j]);
+    const double piv = a[c][c];
+    for (int j = 0; j < 12; ++j) a[c][j] /= piv;
+    for (int r = 0; r < 6; ++r) {
+      if (r == c) continue;
+      const double f = a[r][c];
+      if (f != 0.0)
+        for (int j = 0; j < 12; ++j) a[r][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) cov[i * 6 + j] = -a[i][6 + j];
+  return NDT_OK;
+}
+
+int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out) {
+  if (!h || !out) return NDT_ERR_INVALID_ARGUMENT;
+  *out = h->timing;
+  return NDT_OK;
+}
+
+}  // extern "C"

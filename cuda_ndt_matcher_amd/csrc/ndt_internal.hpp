@@ -1,0 +1,114 @@
+// Host-side state shared by the build, the Newton/scoring launchers and the
+// C ABI.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/ndt_hip.h"
+#include "ndt_device.hpp"
+
+namespace ndt {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t want) {
+    if (want <= bytes && ptr) return hipSuccess;
+    if (ptr) {
+      hipError_t e = hipFree(ptr);
+      if (e != hipSuccess) return e;
+      ptr = nullptr;
+      bytes = 0;
+    }
+    size_t cap = want < 256 ? 256 : want;
+    hipError_t e = hipMalloc(&ptr, cap);
+    if (e == hipSuccess) bytes = cap;
+    return e;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+struct DeviceGrid {
+  // inputs
+  const float* points = nullptr;  // device, 3*n_points
+  DevBuf owned_points;
+  size_t n_points = 0;
+  // sort scratch
+  DevBuf sort_keys_a, sort_keys_b, sort_vals_a, sort_vals_b, hist, status, tile_heads, seg_start, scalars;
+  void* sorted_keys = nullptr;
+  uint32_t* sorted_vals = nullptr;
+  int key_bits = 0;
+  bool key64 = false;
+  int32_t key_min[3] = {0, 0, 0};
+  int key_bits_axis[3] = {0, 0, 0};
+  // voxel slots (one per segment, in Morton order)
+  uint32_t segments = 0;
+  uint32_t num_valid = 0;
+  DevBuf vox, cov, axis, count, coord, valid, table;
+  GridView view{};
+  // host mirrors
+  uint32_t* host_scalars = nullptr;  // pinned, 64 words
+  hipEvent_t ev_build0{}, ev_build1{}, ev_sort0{}, ev_sort1{};
+  float build_ms = 0.f, sort_ms = 0.f;
+  bool ready = false;
+};
+
+int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStream_t st, std::string* err);
+
+// One alignment request inside a persistent launch.
+struct SlotJob {
+  const float* pts;  // device AoS
+  uint32_t n;
+  uint32_t pad;
+  double pose[6];
+};
+
+struct NewtonConfig {
+  double d1, d2;
+  double trans_eps;
+  double step_size;
+  double reg_scale;
+  double reg_t[3];
+  uint32_t max_iter;
+  int32_t search_mode;
+  int32_t metric;
+  int32_t use_ls;
+  int32_t reg_active;  // regularisation enabled and a reference pose set
+  int32_t precision;
+  int32_t wg_per_slot;
+};
+
+static_assert(sizeof(ndt_result) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8, "ndt_result layout");
+
+// Launches the persistent Newton kernel over `slots` jobs (device array) and
+// writes ndt_result per slot (device array).  ws must hold
+// newton_workspace_bytes(slots, wg_per_slot).
+size_t newton_workspace_bytes(int slots, int wg_per_slot);
+hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
+                         ndt_result* d_out, void* ws, hipStream_t st, bool cooperative);
+int newton_max_coresident_workgroups();
+
+// One derivative pass (score, gradient, Hessian, count) at a pose.
+hipError_t launch_derivatives(const GridView& g, const NewtonConfig& cfg, const float* d_pts, uint32_t n,
+                              const double pose[6], int hess, double* d_partials, int blocks, double* d_out,
+                              hipStream_t st);
+
+enum ScoreKind { kScoreNvtl = 0, kScoreTp = 1, kScorePerPoint = 2, kScoreNvtlCell27 = 3 };
+// Scores `m` poses over n points.  For NVTL/TP writes out[m]; per-point writes out[n].
+hipError_t launch_score(const GridView& g, double d1, double d2, int kind, const float* d_pts, uint32_t n,
+                        const double* d_poses, int m, double* d_partials, int blocks_per_pose, double* d_out,
+                        hipStream_t st);
+
+hipError_t launch_lookup(const GridView& g, int mode, const float* d_q, uint32_t n, uint32_t max_k,
+                         uint32_t* d_counts, int32_t* d_idx, hipStream_t st);
+
+}  // namespace ndt

@@ -1,0 +1,803 @@
+// Persistent Newton kernel and scoring kernels (gfx950).
+//
+// One launch runs whole alignments (NdtOptimizer::align,
+// src/ndt_cuda/src/optimization/solver.rs:135-305) for `slots` independent
+// requests.  Each slot owns `wg_per_slot` workgroups.  Per Newton iteration:
+//   - every thread transforms its points (f64), finds the neighbouring voxels
+//     through the hash, and accumulates score, gradient and the lower
+//     triangle of the Hessian in registers (compute_derivatives_cpu_with_metric,
+//     derivatives/cpu.rs:417-538);
+//   - the workgroup reduces them with wave shuffles and LDS;
+//   - with more than one workgroup per slot, the workgroups publish partials
+//     and meet at a slot barrier, then every workgroup sums the partials in
+//     rank order, so all of them hold bit-identical totals;
+//   - thread 0 of every workgroup runs the same FP64 Newton step
+//     (newton.rs:160-176), convergence test, step clamp or More-Thuente
+//     line search (more_thuente.rs:268-420) and pose update, so no second
+//     barrier is needed.
+// After the loop the slot computes NVTL at the final pose with the Isometry
+// transform (scoring/nvtl.rs:211-268) and rank 0 writes the result.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ndt_device.hpp"
+#include "ndt_internal.hpp"
+
+namespace ndt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxIter = 256;
+constexpr int kPartialStride = 32;  // doubles per workgroup partial
+
+// ---------------------------------------------------------------------------
+// More-Thuente line search as a state machine fed one (score, derivative)
+// evaluation at a time (optimization/more_thuente.rs:268-420, defaults
+// step_max 0.1, step_min 1e-9, mu 1e-4, nu 0.9, 10 iterations).
+struct MTState {
+  double phi0, dphi0, a_t, phi_t, dphi_t, psi_t, dpsi_t, a_l, f_l, g_l, a_u, f_u, g_u;
+  int open, it, phase, done, conv;
+};
+
+constexpr double kMtMax = 0.1, kMtMin = 1e-9, kMtMu = 1e-4, kMtNu = 0.9;
+constexpr int kMtIters = 10;
+
+__device__ double mt_trial(double a_l, double f_l, double g_l, double a_u, double f_u, double g_u, double a_t,
+                           double f_t, double g_t) {
+  if (f_t > f_l) {
+    const double z = 3.0 * (f_t - f_l) / (a_t - a_l) - g_t - g_l;
+    const double w_sq = z * z - g_t * g_l;
+    if (w_sq < 0.0) return a_l - 0.5 * (a_l - a_t) * g_l / (g_l - (f_l - f_t) / (a_l - a_t));
+    const double w = sqrt(w_sq);
+    const double a_c = a_l + (a_t - a_l) * (w - g_l - z) / (g_t - g_l + 2.0 * w);
+    const double denom = g_l - (f_l - f_t) / (a_l - a_t);
+    if (fabs(denom) < 1e-15) return a_c;
+    const double a_q = a_l - 0.5 * (a_l - a_t) * g_l / denom;
+    if (fabs(a_c - a_l) < fabs(a_q - a_l)) return a_c;
+    return 0.5 * (a_q + a_c);
+  }
+  if (g_t * g_l < 0.0) {
+    const double z = 3.0 * (f_t - f_l) / (a_t - a_l) - g_t - g_l;
+    const double w_sq = z * z - g_t * g_l;
+    if (w_sq < 0.0) return a_l - (a_l - a_t) / (g_l - g_t) * g_l;
+    const double w = sqrt(w_sq);
+    const double a_c = a_l + (a_t - a_l) * (w - g_l - z) / (g_t - g_l + 2.0 * w);
+    const double a_s = a_l - (a_l - a_t) / (g_l - g_t) * g_l;
+    return (fabs(a_c - a_t) >= fabs(a_s - a_t)) ? a_c : a_s;
+  }
+  if (fabs(g_t) <= fabs(g_l)) {
+    const double z = 3.0 * (f_t - f_l) / (a_t - a_l) - g_t - g_l;
+    const double w_sq = z * z - g_t * g_l;
+    double a_c;
+    if (w_sq >= 0.0) {
+      const double w = sqrt(w_sq);
+      a_c = a_l + (a_t - a_l) * (w - g_l - z) / (g_t - g_l + 2.0 * w);
+    } else {
+      a_c = (a_t > a_l) ? a_t + 0.66 * (a_u - a_t) : a_t - 0.66 * (a_t - a_u);
+    }
+    const double a_s = (fabs(g_l - g_t) > 1e-15) ? a_l - (a_l - a_t) / (g_l - g_t) * g_l : a_c;
+    const double nxt = (fabs(a_c - a_t) < fabs(a_s - a_t)) ? a_c : a_s;
+    if (a_t > a_l) return fmin(a_t + 0.66 * (a_u - a_t), nxt);
+    return fmax(a_t + 0.66 * (a_u - a_t), nxt);
+  }
+  const double z = 3.0 * (f_t - f_u) / (a_t - a_u) - g_t - g_u;
+  const double w_sq = z * z - g_t * g_u;
+  if (w_sq < 0.0) return 0.5 * (a_l + a_u);
+  const double w = sqrt(w_sq);
+  return a_u + (a_t - a_u) * (w - g_u - z) / (g_t - g_u + 2.0 * w);
+}
+
+__device__ bool mt_update(MTState& s, double f_t, double g_t) {
+  if (f_t > s.f_l) {
+    s.a_u = s.a_t; s.f_u = f_t; s.g_u = g_t;
+    return false;
+  }
+  if (g_t * (s.a_l - s.a_t) > 0.0) {
+    s.a_l = s.a_t; s.f_l = f_t; s.g_l = g_t;
+    return false;
+  }
+  if (g_t * (s.a_l - s.a_t) < 0.0) {
+    s.a_u = s.a_l; s.f_u = s.f_l; s.g_u = s.g_l;
+    s.a_l = s.a_t; s.f_l = f_t; s.g_l = g_t;
+    return false;
+  }
+  return true;
+}
+
+__device__ inline double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ void mt_begin(MTState& s, double score0, double deriv0, double init_step) {
+  s.phi0 = -score0;
+  s.dphi0 = -deriv0;
+  s.a_t = clampd(init_step, kMtMin, kMtMax);
+  s.phase = 0;
+  s.done = 0;
+  s.conv = 0;
+}
+
+// Consumes the evaluation at s.a_t; either finishes or sets the next a_t.
+__device__ void mt_feed(MTState& s, double score_t, double deriv_t) {
+  s.phi_t = -score_t;
+  s.dphi_t = -deriv_t;
+  if (s.phase == 0) {
+    s.a_l = 0.0;
+    s.a_u = 0.0;
+    s.f_l = s.phi0 - s.phi0 - kMtMu * s.dphi0 * 0.0;
+    s.g_l = s.dphi0 - kMtMu * s.dphi0;
+    s.f_u = s.f_l;
+    s.g_u = s.g_l;
+    s.open = 1;
+    s.it = 0;
+    s.psi_t = s.phi_t - s.phi0 - kMtMu * s.dphi0 * s.a_t;
+    s.dpsi_t = s.dphi_t - kMtMu * s.dphi0;
+  } else {
+    s.psi_t = s.phi_t - s.phi0 - kMtMu * s.dphi0 * s.a_t;
+    s.dpsi_t = s.dphi_t - kMtMu * s.dphi0;
+    if (s.open && s.psi_t <= 0.0 && s.dpsi_t >= 0.0) {
+      s.open = 0;
+      s.f_l = s.f_l + s.phi0 - kMtMu * s.dphi0 * s.a_l;
+      s.g_l += kMtMu * s.dphi0;
+      s.f_u = s.f_u + s.phi0 - kMtMu * s.dphi0 * s.a_u;
+      s.g_u += kMtMu * s.dphi0;
+    }
+    const bool stop = s.open ? mt_update(s, s.psi_t, s.dpsi_t) : mt_update(s, s.phi_t, s.dphi_t);
+    if (stop) {
+      s.done = 1;
+      s.conv = 0;
+      return;
+    }
+    s.it += 1;
+  }
+  if (s.it >= kMtIters) {
+    s.done = 1;
+    s.conv = 0;
+    return;
+  }
+  if (s.psi_t <= 0.0 && fabs(s.dphi_t) <= kMtNu * fabs(s.dphi0)) {
+    s.done = 1;
+    s.conv = 1;
+    return;
+  }
+  const double nt = s.open ? mt_trial(s.a_l, s.f_l, s.g_l, s.a_u, s.f_u, s.g_u, s.a_t, s.psi_t, s.dpsi_t)
+                           : mt_trial(s.a_l, s.f_l, s.g_l, s.a_u, s.f_u, s.g_u, s.a_t, s.phi_t, s.dphi_t);
+  s.a_t = clampd(nt, kMtMin, kMtMax);
+  s.phase = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Per-point work.
+__device__ inline void point_jacobian(const PoseTables& T, double px, double py, double pz, double Jr[3][3],
+                                      double ht[15], bool hess) {
+  double gt[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) gt[r] = T.j[r][0] * px + T.j[r][1] * py + T.j[r][2] * pz;
+  Jr[0][0] = 0.0;   Jr[1][0] = gt[0]; Jr[2][0] = gt[1];
+  Jr[0][1] = gt[2]; Jr[1][1] = gt[3]; Jr[2][1] = gt[4];
+  Jr[0][2] = gt[5]; Jr[1][2] = gt[6]; Jr[2][2] = gt[7];
+  if (hess) {
+#pragma unroll
+    for (int r = 0; r < 15; ++r) ht[r] = T.h[r][0] * px + T.h[r][1] * py + T.h[r][2] * pz;
+  } else {
+#pragma unroll
+    for (int r = 0; r < 15; ++r) ht[r] = 0.0;
+  }
+}
+
+// Accumulates the derivative contributions of points [first, n) step `stride`.
+// acc: 28 doubles (score, grad, H lower), corr: pair count.
+template <bool kHess>
+__device__ inline void derivative_points(const GridView& g, const NewtonConfig& cfg, const PoseTables& T,
+                                         const float* __restrict__ pts, uint32_t n, uint32_t first,
+                                         uint32_t stride, double* acc, uint32_t& corr) {
+  for (uint32_t i = first; i < n; i += stride) {
+    const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+    double t[3];
+    t[0] = T.R[0] * px + T.R[1] * py + T.R[2] * pz + T.t[0];
+    t[1] = T.R[3] * px + T.R[4] * py + T.R[5] * pz + T.t[1];
+    t[2] = T.R[6] * px + T.R[7] * py + T.R[8] * pz + T.t[2];
+    double Jr[3][3], ht[15];
+    point_jacobian(T, px, py, pz, Jr, ht, kHess);
+    for_each_neighbor(g, cfg.search_mode, (float)t[0], (float)t[1], (float)t[2], [&](int32_t s, const float4 m0) {
+      const float4 m1 = g.vox[3 * s + 1];
+      const float4 m2 = g.vox[3 * s + 2];
+      const double x[3] = {t[0] - (double)m0.x, t[1] - (double)m0.y, t[2] - (double)m0.z};
+      corr += 1;
+      if (cfg.metric == 1) {
+        const double nn[3] = {(double)g.axis[3 * s], (double)g.axis[3 * s + 1], (double)g.axis[3 * s + 2]};
+        pair_p2plane(x, nn, Jr, ht, cfg.d1, cfg.d2, kHess, acc);
+      } else {
+        const double C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
+        pair_p2d(x, C, Jr, ht, cfg.d1, cfg.d2, kHess, acc);
+      }
+    });
+  }
+}
+
+__device__ inline double point_score(const double x[3], const float4& m0, const float4& m1, const float4& m2,
+                                     double d1, double d2) {
+  // compute_point_voxel_score (scoring/nvtl.rs:196-206)
+  const double d[3] = {x[0] - (double)m0.x, x[1] - (double)m0.y, x[2] - (double)m0.z};
+  const double C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
+  double Cd[3];
+  for (int r = 0; r < 3; ++r) Cd[r] = C[3 * r] * d[0] + C[3 * r + 1] * d[1] + C[3 * r + 2] * d[2];
+  const double m = d[0] * Cd[0] + d[1] * Cd[1] + d[2] * Cd[2];
+  return -d1 * exp(-d2 * m / 2.0);
+}
+
+// NVTL contributions (max per point, points with neighbours) over a strided range.
+__device__ inline void nvtl_points(const GridView& g, double d1, double d2, const double q[4], const double tr[3],
+                                   const float* __restrict__ pts, uint32_t n, uint32_t first, uint32_t stride,
+                                   double& sum, double& cnt) {
+  for (uint32_t i = first; i < n; i += stride) {
+    double x[3];
+    iso_transform(q, tr, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], x);
+    double best = -HUGE_VAL;
+    bool any = false;
+    for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, const float4 m0) {
+      const double sc = point_score(x, m0, g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
+      best = fmax(best, sc);
+      any = true;
+    });
+    if (any) {
+      sum += best;
+      cnt += 1.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+struct SlotState {
+  double pose[6];
+  double best_pose[6];
+  double dir[6];
+  double last_H[36];
+  double best_score;
+  double step;
+  double final_pose[6];
+  double final_score;
+  double final_H[36];
+  double hist[3 * (kMaxIter + 1)];
+  MTState mt;
+  int nhist;
+  int status;       // -1 running
+  int iterations;
+  int ctrl;         // 0: take a step, 1: stop, 2: line search
+  uint32_t last_corr, final_corr;
+};
+
+__device__ inline void slot_barrier(uint32_t* counter, uint32_t target) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(2);
+  }
+  __syncthreads();
+  __threadfence();
+}
+
+// Reduces `count` per-thread values across the slot.  Result in tot[0..count)
+// (LDS), identical in every workgroup of the slot.
+__device__ inline void slot_reduce(const double* vals, int count, double* lds_scratch, double* wg_tot, double* tot,
+                                   double* partials, uint32_t* counter, uint32_t& gen, int W, int rank) {
+  block_sum<kBlock>(vals, count, lds_scratch, wg_tot);
+  if (W == 1) {
+    if (threadIdx.x < (unsigned)count) tot[threadIdx.x] = wg_tot[threadIdx.x];
+    __syncthreads();
+    return;
+  }
+  double* buf = partials + (size_t)(gen & 1) * W * kPartialStride;
+  if (threadIdx.x < (unsigned)count) buf[rank * kPartialStride + threadIdx.x] = wg_tot[threadIdx.x];
+  ++gen;
+  slot_barrier(counter, gen * (uint32_t)W);
+  if (threadIdx.x < (unsigned)count) {
+    double s = 0.0;
+    for (int r = 0; r < W; ++r)
+      s += __hip_atomic_load(buf + r * kPartialStride + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+__device__ void regularization_terms(const NewtonConfig& cfg, const double* pose, uint32_t corr, double* rs,
+                                     double rg[6], double rH[36]) {
+  // RegularizationTerm::compute_derivatives (optimization/regularization.rs:102-159)
+  *rs = 0.0;
+  for (int i = 0; i < 6; ++i) rg[i] = 0.0;
+  for (int i = 0; i < 36; ++i) rH[i] = 0.0;
+  if (!cfg.reg_active) return;
+  const double scale = cfg.reg_scale, w = (double)corr;
+  const double dx = cfg.reg_t[0] - pose[0], dy = cfg.reg_t[1] - pose[1];
+  double sy, cy;
+  sincos(pose[5], &sy, &cy);
+  const double lon = dy * sy + dx * cy;
+  *rs = -scale * w * lon * lon;
+  rg[0] = scale * w * 2.0 * cy * lon;
+  rg[1] = scale * w * 2.0 * sy * lon;
+  rH[0] = -scale * w * 2.0 * cy * cy;
+  rH[1] = -scale * w * 2.0 * cy * sy;
+  rH[6] = rH[1];
+  rH[7] = -scale * w * 2.0 * sy * sy;
+}
+
+__device__ uint32_t oscillation_count(const double* h, int k) {
+  // count_oscillation (optimization/oscillation.rs:50-99)
+  if (k < 3) return 0;
+  uint32_t cnt = 0, mx = 0;
+  for (int i = 2; i < k; ++i) {
+    double cur[3], prev[3];
+    for (int a = 0; a < 3; ++a) {
+      cur[a] = h[3 * i + a] - h[3 * (i - 1) + a];
+      prev[a] = h[3 * (i - 1) + a] - h[3 * (i - 2) + a];
+    }
+    const double cn = sqrt(cur[0] * cur[0] + cur[1] * cur[1] + cur[2] * cur[2]);
+    const double pn = sqrt(prev[0] * prev[0] + prev[1] * prev[1] + prev[2] * prev[2]);
+    if (cn < 1e-10 || pn < 1e-10) {
+      cnt = 0;
+      continue;
+    }
+    double c = 0.0;
+    for (int a = 0; a < 3; ++a) c += (cur[a] / cn) * (prev[a] / pn);
+    cnt = (c < -0.9) ? cnt + 1 : 0;
+    mx = cnt > mx ? cnt : mx;
+  }
+  return mx;
+}
+
+__global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig cfg, const SlotJob* __restrict__ jobs,
+                                                        ndt_result* __restrict__ out, double* __restrict__ partials,
+                                                        uint32_t* __restrict__ counters) {
+  __shared__ PoseTables s_tab;
+  __shared__ SlotState S;
+  __shared__ double s_scratch[(kBlock / kWave) * kAccSlots];
+  __shared__ double s_wg[kAccSlots];
+  __shared__ double s_tot[kAccSlots];
+  __shared__ double s_q[4];
+
+  const int W = cfg.wg_per_slot;
+  const int slot = blockIdx.x / W;
+  const int rank = blockIdx.x % W;
+  const SlotJob job = jobs[slot];
+  double* my_partials = partials + (size_t)slot * 2 * W * kPartialStride;
+  uint32_t* my_counter = counters + slot;
+  uint32_t gen = 0;
+  const uint32_t first = rank * kBlock + threadIdx.x;
+  const uint32_t stride = (uint32_t)W * kBlock;
+  const int max_iter = cfg.max_iter < (uint32_t)kMaxIter ? (int)cfg.max_iter : kMaxIter;
+
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 6; ++i) {
+      S.pose[i] = job.pose[i];
+      S.best_pose[i] = job.pose[i];
+    }
+    for (int i = 0; i < 36; ++i) S.last_H[i] = 0.0;
+    S.best_score = -HUGE_VAL;
+    S.hist[0] = job.pose[0];
+    S.hist[1] = job.pose[1];
+    S.hist[2] = job.pose[2];
+    S.nhist = 1;
+    S.status = -1;
+    S.last_corr = 0;
+    make_pose_tables(S.pose, &s_tab);
+  }
+  __syncthreads();
+
+  for (int it = 0; it < max_iter; ++it) {
+    double acc[kAccSlots];
+#pragma unroll
+    for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
+    uint32_t corr = 0;
+    derivative_points<true>(g, cfg, s_tab, job.pts, job.n, first, stride, acc, corr);
+    acc[kAccVals] = (double)corr;
+    slot_reduce(acc, kAccVals + 1, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank);
+
+    if (threadIdx.x == 0) {
+      const uint32_t c = (uint32_t)s_tot[kAccVals];
+      S.ctrl = 0;
+      if (c < 10) {  // solver.rs:168-174
+        if (it == 0) {
+          S.status = NDT_NO_CORRESPONDENCES;
+          S.ctrl = 1;
+        } else {
+          S.ctrl = 3;  // break: max-iterations result
+        }
+      } else {
+        S.last_corr = c;
+        double rs, rg[6], rH[36];
+        regularization_terms(cfg, S.pose, c, &rs, rg, rH);
+        const double score = s_tot[0] + rs;
+        double grad[6], H[36];
+        for (int i = 0; i < 6; ++i) grad[i] = s_tot[1 + i] + rg[i];
+        for (int i = 0; i < 6; ++i)
+          for (int j = 0; j <= i; ++j) {
+            const double v = s_tot[7 + tri(i, j)];
+            H[i * 6 + j] = v + rH[i * 6 + j];
+            H[j * 6 + i] = v + rH[j * 6 + i];
+          }
+        for (int i = 0; i < 36; ++i) S.last_H[i] = H[i];
+        if (score > S.best_score) {
+          S.best_score = score;
+          for (int i = 0; i < 6; ++i) S.best_pose[i] = S.pose[i];
+        }
+        double delta[6];
+        if (!newton_step_device(grad, H, delta)) {
+          S.status = NDT_SINGULAR_HESSIAN;
+          for (int i = 0; i < 6; ++i) S.final_pose[i] = S.best_pose[i];
+          S.final_score = S.best_score;
+          S.iterations = it;
+          for (int i = 0; i < 36; ++i) S.final_H[i] = S.last_H[i];
+          S.final_corr = S.last_corr;
+          S.ctrl = 1;
+        } else {
+          double norm = 0.0;
+          for (int i = 0; i < 6; ++i) norm += delta[i] * delta[i];
+          norm = sqrt(norm);
+          if (norm < cfg.trans_eps) {  // solver.rs:229-253
+            S.status = NDT_CONVERGED;
+            for (int i = 0; i < 6; ++i) S.final_pose[i] = S.pose[i];
+            S.final_score = score;
+            S.iterations = it + 1;
+            for (int i = 0; i < 36; ++i) S.final_H[i] = H[i];
+            S.final_corr = c;
+            S.ctrl = 1;
+          } else {
+            double gd = 0.0;
+            for (int i = 0; i < 6; ++i) {
+              S.dir[i] = delta[i] / norm;
+              gd += grad[i] * S.dir[i];
+            }
+            if (gd <= 0.0)
+              for (int i = 0; i < 6; ++i) S.dir[i] = -S.dir[i];
+            if (cfg.use_ls) {
+              double d0 = 0.0;
+              for (int i = 0; i < 6; ++i) d0 += grad[i] * S.dir[i];
+              if (d0 <= 0.0) {
+                S.step = cfg.step_size;
+              } else {
+                mt_begin(S.mt, score, d0, cfg.step_size);
+                S.ctrl = 2;
+              }
+            } else {
+              S.step = fmin(norm, cfg.step_size);
+            }
+          }
+        }
+      }
+      if (S.ctrl == 2) {
+        double tp[6];
+        for (int i = 0; i < 6; ++i) tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
+        make_pose_tables(tp, &s_tab);
+      }
+    }
+    __syncthreads();
+    const int ctrl = S.ctrl;
+    if (ctrl == 1 || ctrl == 3) break;
+    if (ctrl == 2) {
+      // line_search_with_regularization (solver.rs:640-699): phi(a) = score and
+      // directional derivative at pose + a*dir, without the Hessian.
+      for (;;) {
+        double la[kAccSlots];
+#pragma unroll
+        for (int i = 0; i < kAccSlots; ++i) la[i] = 0.0;
+        uint32_t lc = 0;
+        derivative_points<false>(g, cfg, s_tab, job.pts, job.n, first, stride, la, lc);
+        la[7] = (double)lc;
+        slot_reduce(la, 8, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank);
+        if (threadIdx.x == 0) {
+          double tp[6];
+          for (int i = 0; i < 6; ++i) tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
+          double rs, rg[6], rH[36];
+          regularization_terms(cfg, tp, (uint32_t)s_tot[7], &rs, rg, rH);
+          double dd = 0.0;
+          for (int i = 0; i < 6; ++i) dd += (s_tot[1 + i] + rg[i]) * S.dir[i];
+          mt_feed(S.mt, s_tot[0] + rs, dd);
+          if (S.mt.done) {
+            S.step = S.mt.conv ? S.mt.a_t : kMtMax;
+          } else {
+            for (int i = 0; i < 6; ++i) tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
+            make_pose_tables(tp, &s_tab);
+          }
+        }
+        __syncthreads();
+        if (S.mt.done) break;
+      }
+    }
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < 6; ++i) S.pose[i] = S.pose[i] + S.dir[i] * S.step;  // types.rs:183-197
+      S.hist[3 * S.nhist + 0] = S.pose[0];
+      S.hist[3 * S.nhist + 1] = S.pose[1];
+      S.hist[3 * S.nhist + 2] = S.pose[2];
+      S.nhist += 1;
+      make_pose_tables(S.pose, &s_tab);
+    }
+    __syncthreads();
+  }
+
+  if (threadIdx.x == 0 && S.status < 0) {  // max iterations (solver.rs:286-304)
+    S.status = NDT_MAX_ITERATIONS;
+    for (int i = 0; i < 6; ++i) S.final_pose[i] = S.best_pose[i];
+    S.final_score = S.best_score;
+    S.iterations = (int)cfg.max_iter;
+    for (int i = 0; i < 36; ++i) S.final_H[i] = S.last_H[i];
+    S.final_corr = S.last_corr;
+  }
+  if (threadIdx.x == 0) quat_from_euler(S.final_pose[3], S.final_pose[4], S.final_pose[5], s_q);
+  __syncthreads();
+
+  double nv_sum = 0.0, nv_cnt = 0.0;
+  const bool no_corr = S.status == NDT_NO_CORRESPONDENCES;
+  if (!no_corr) nvtl_points(g, cfg.d1, cfg.d2, s_q, S.final_pose, job.pts, job.n, first, stride, nv_sum, nv_cnt);
+  double nv[2] = {nv_sum, nv_cnt};
+  slot_reduce(nv, 2, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank);
+
+  if (rank == 0 && threadIdx.x == 0) {
+    ndt_result r;
+    if (no_corr) {  // NdtResult::no_correspondences (optimization/types.rs:133-145)
+      for (int i = 0; i < 6; ++i) r.pose[i] = job.pose[i];
+      r.status = NDT_NO_CORRESPONDENCES;
+      r.converged = 0;
+      r.score = 0.0;
+      r.transform_probability = 0.0;
+      r.nvtl = 0.0;
+      r.iterations = 0;
+      r.num_correspondences = 0;
+      r.oscillation_count = 0;
+      for (int i = 0; i < 36; ++i) r.hessian[i] = 0.0;
+    } else {
+      for (int i = 0; i < 6; ++i) r.pose[i] = S.final_pose[i];
+      r.status = S.status;
+      r.converged = S.status == NDT_CONVERGED;
+      r.score = S.final_score;
+      r.transform_probability = job.n > 0 ? S.final_score / (double)job.n : 0.0;
+      r.nvtl = s_tot[1] > 0.0 ? s_tot[0] / s_tot[1] : 0.0;
+      r.iterations = (uint32_t)S.iterations;
+      r.num_correspondences = S.final_corr;
+      r.oscillation_count = oscillation_count(S.hist, S.nhist);
+      for (int i = 0; i < 36; ++i) r.hessian[i] = S.final_H[i];
+    }
+    r.reserved = 0;
+    out[slot] = r;
+  }
+}
+
+__global__ void zero_u32(uint32_t* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Single derivative pass (parity entry point).
+__global__ __launch_bounds__(kBlock) void derivative_kernel(GridView g, NewtonConfig cfg, const float* __restrict__ pts,
+                                                            uint32_t n, PoseTables T, int hess,
+                                                            double* __restrict__ partials) {
+  __shared__ double s_scratch[(kBlock / kWave) * kAccSlots];
+  __shared__ double s_wg[kAccSlots];
+  double acc[kAccSlots];
+#pragma unroll
+  for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
+  uint32_t corr = 0;
+  const uint32_t first = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+  if (hess)
+    derivative_points<true>(g, cfg, T, pts, n, first, stride, acc, corr);
+  else
+    derivative_points<false>(g, cfg, T, pts, n, first, stride, acc, corr);
+  acc[kAccVals] = (double)corr;
+  block_sum<kBlock>(acc, kAccVals + 1, s_scratch, s_wg);
+  if (threadIdx.x < kAccVals + 1) partials[blockIdx.x * kPartialStride + threadIdx.x] = s_wg[threadIdx.x];
+}
+
+__global__ void reduce_partials(const double* __restrict__ partials, int blocks, int count, double* __restrict__ out) {
+  // out[pose * count + v] = sum over blocks, fixed order; grid.y = pose.
+  const int v = threadIdx.x;
+  if (v >= count) return;
+  const double* p = partials + (size_t)blockIdx.y * blocks * kPartialStride;
+  double s = 0.0;
+  for (int b = 0; b < blocks; ++b) s += p[b * kPartialStride + v];
+  out[blockIdx.y * count + v] = s;
+}
+
+// Scoring: kind 0 NVTL (radius, max), 1 TP (radius, sum over pairs),
+// 2 per-point sums, 3 NVTL with the 3x3x3 cell scan.  grid = (blocks, poses).
+__global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, double d2, int kind,
+                                                       const float* __restrict__ pts, uint32_t n,
+                                                       const double* __restrict__ poses, double* __restrict__ partials,
+                                                       double* __restrict__ per_point) {
+  __shared__ double s_scratch[(kBlock / kWave) * 2];
+  __shared__ double s_wg[2];
+  __shared__ double s_q[4];
+  __shared__ double s_t[3];
+  const double* pose = poses + 6 * blockIdx.y;
+  if (threadIdx.x == 0) {
+    quat_from_euler(pose[3], pose[4], pose[5], s_q);
+    s_t[0] = pose[0];
+    s_t[1] = pose[1];
+    s_t[2] = pose[2];
+  }
+  __syncthreads();
+  double sum = 0.0, cnt = 0.0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double x[3];
+    iso_transform(s_q, s_t, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], x);
+    if (kind == kScoreNvtl || kind == kScoreNvtlCell27) {
+      double best = -HUGE_VAL;
+      bool any = false;
+      auto vis = [&](int32_t s, const float4 m0) {
+        best = fmax(best, point_score(x, m0, g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2));
+        any = true;
+      };
+      if (kind == kScoreNvtl)
+        for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], vis);
+      else
+        for_each_cell27(g, x[0], x[1], x[2], vis);
+      if (any) {
+        sum += best;
+        cnt += 1.0;
+      }
+    } else {
+      double ps = 0.0;
+      double pc = 0.0;
+      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, const float4 m0) {
+        ps += point_score(x, m0, g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
+        pc += 1.0;
+      });
+      if (kind == kScorePerPoint) {
+        per_point[i] = ps;
+      } else {
+        sum += ps;
+        cnt += pc;
+      }
+    }
+  }
+  if (kind == kScorePerPoint) return;
+  const double v[2] = {sum, cnt};
+  block_sum<kBlock>(v, 2, s_scratch, s_wg);
+  if (threadIdx.x < 2)
+    partials[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPartialStride + threadIdx.x] = s_wg[threadIdx.x];
+}
+
+__global__ void finish_scores(const double* __restrict__ sums, int m, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double s = sums[2 * i], c = sums[2 * i + 1];
+  out[i] = c > 0.0 ? s / c : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, const float* __restrict__ q, uint32_t n,
+                                                        uint32_t max_k, uint32_t* __restrict__ counts,
+                                                        int32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = 0;
+  auto vis = [&](int32_t s, const float4) {
+    if (c < max_k) idx[(size_t)i * max_k + c] = s;
+    ++c;
+  };
+  if (mode == 4)
+    for_each_cell27(g, (double)q[3 * i], (double)q[3 * i + 1], (double)q[3 * i + 2], vis);
+  else
+    for_each_neighbor(g, mode, q[3 * i], q[3 * i + 1], q[3 * i + 2], vis);
+  for (uint32_t k = c; k < max_k; ++k) idx[(size_t)i * max_k + k] = -1;
+  counts[i] = c;
+}
+
+}  // namespace
+
+size_t newton_workspace_bytes(int slots, int wg_per_slot) {
+  const size_t partial = (size_t)slots * 2 * wg_per_slot * kPartialStride * sizeof(double);
+  return partial + (size_t)slots * sizeof(uint32_t) + 256;
+}
+
+int newton_max_coresident_workgroups() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(newton_kernel), kBlock,
+                                                   0) != hipSuccess)
+    return 0;
+  return cus * per_cu;
+}
+
+hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
+                         ndt_result* d_out, void* ws, hipStream_t st, bool cooperative) {
+  const int W = cfg.wg_per_slot;
+  double* partials = reinterpret_cast<double*>(ws);
+  uint32_t* counters =
+      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
+  hipLaunchKernelGGL(zero_u32, dim3((slots + 255) / 256), dim3(256), 0, st, counters, slots);
+  const dim3 grid((unsigned)(slots * W)), block(kBlock);
+  if (cooperative) {
+    GridView gg = g;
+    NewtonConfig cc = cfg;
+    const SlotJob* jj = d_jobs;
+    ndt_result* oo = d_out;
+    void* args[] = {&gg, &cc, &jj, &oo, &partials, &counters};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(newton_kernel), grid, block, args, 0, st);
+  }
+  hipLaunchKernelGGL(newton_kernel, grid, block, 0, st, g, cfg, d_jobs, d_out, partials, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_derivatives(const GridView& g, const NewtonConfig& cfg, const float* d_pts, uint32_t n,
+                              const double pose[6], int hess, double* d_partials, int blocks, double* d_out,
+                              hipStream_t st) {
+  PoseTables T;
+  // Host copy of make_pose_tables: computed on the host with the same formulas.
+  {
+    const double sr = sin(pose[3]), cr = cos(pose[3]), sp = sin(pose[4]), cp = cos(pose[4]), sy = sin(pose[5]),
+                 cy = cos(pose[5]);
+    T.t[0] = pose[0];
+    T.t[1] = pose[1];
+    T.t[2] = pose[2];
+    T.R[0] = cp * cy;                T.R[1] = -cp * sy;               T.R[2] = sp;
+    T.R[3] = sr * sp * cy + cr * sy; T.R[4] = cr * cy - sr * sp * sy; T.R[5] = -sr * cp;
+    T.R[6] = sr * sy - cr * sp * cy; T.R[7] = cr * sp * sy + sr * cy; T.R[8] = cr * cp;
+    auto snap = [](double a, double* s, double* c) {
+      if (fabs(a) < 1e-5) {
+        *s = 0.0;
+        *c = 1.0;
+      } else {
+        *s = sin(a);
+        *c = cos(a);
+      }
+    };
+    double sx, cx, sy2, cy2, sz, cz;
+    snap(pose[3], &sx, &cx);
+    snap(pose[4], &sy2, &cy2);
+    snap(pose[5], &sz, &cz);
+    const double s_y = sy2, c_y = cy2;
+    auto& J = T.j;
+    J[0][0] = -sx * sz + cx * s_y * cz; J[0][1] = -sx * cz - cx * s_y * sz; J[0][2] = -cx * c_y;
+    J[1][0] = cx * sz + sx * s_y * cz;  J[1][1] = cx * cz - sx * s_y * sz;  J[1][2] = -sx * c_y;
+    J[2][0] = -s_y * cz;                J[2][1] = s_y * sz;                 J[2][2] = c_y;
+    J[3][0] = sx * c_y * cz;            J[3][1] = -sx * c_y * sz;           J[3][2] = sx * s_y;
+    J[4][0] = -cx * c_y * cz;           J[4][1] = cx * c_y * sz;            J[4][2] = -cx * s_y;
+    J[5][0] = -c_y * sz;                J[5][1] = -c_y * cz;                J[5][2] = 0.0;
+    J[6][0] = cx * cz - sx * s_y * sz;  J[6][1] = -cx * sz - sx * s_y * cz; J[6][2] = 0.0;
+    J[7][0] = sx * cz + cx * s_y * sz;  J[7][1] = cx * s_y * cz - sx * sz;  J[7][2] = 0.0;
+    auto& H = T.h;
+    H[0][0] = -cx * sz - sx * s_y * cz;  H[0][1] = -cx * cz + sx * s_y * sz;  H[0][2] = sx * c_y;
+    H[1][0] = -sx * sz + cx * s_y * cz;  H[1][1] = -cx * s_y * sz - sx * cz;  H[1][2] = -cx * c_y;
+    H[2][0] = cx * c_y * cz;             H[2][1] = -cx * c_y * sz;            H[2][2] = cx * s_y;
+    H[3][0] = sx * c_y * cz;             H[3][1] = -sx * c_y * sz;            H[3][2] = sx * s_y;
+    H[4][0] = -sx * cz - cx * s_y * sz;  H[4][1] = sx * sz - cx * s_y * cz;   H[4][2] = 0.0;
+    H[5][0] = cx * cz - sx * s_y * sz;   H[5][1] = -sx * s_y * cz - cx * sz;  H[5][2] = 0.0;
+    H[6][0] = -c_y * cz;                 H[6][1] = c_y * sz;                  H[6][2] = s_y;
+    H[7][0] = -sx * s_y * cz;            H[7][1] = sx * s_y * sz;             H[7][2] = sx * c_y;
+    H[8][0] = cx * s_y * cz;             H[8][1] = -cx * s_y * sz;            H[8][2] = -cx * c_y;
+    H[9][0] = s_y * sz;                  H[9][1] = s_y * cz;                  H[9][2] = 0.0;
+    H[10][0] = -sx * c_y * sz;           H[10][1] = -sx * c_y * cz;           H[10][2] = 0.0;
+    H[11][0] = cx * c_y * sz;            H[11][1] = cx * c_y * cz;            H[11][2] = 0.0;
+    H[12][0] = -c_y * cz;                H[12][1] = c_y * sz;                 H[12][2] = 0.0;
+    H[13][0] = -cx * sz - sx * s_y * cz; H[13][1] = -cx * cz + sx * s_y * sz; H[13][2] = 0.0;
+    H[14][0] = -sx * sz + cx * s_y * cz; H[14][1] = -cx * s_y * sz - sx * cz; H[14][2] = 0.0;
+  }
+  hipLaunchKernelGGL(derivative_kernel, dim3(blocks), dim3(kBlock), 0, st, g, cfg, d_pts, n, T, hess, d_partials);
+  hipLaunchKernelGGL(reduce_partials, dim3(1, 1), dim3(64), 0, st, d_partials, blocks, kAccVals + 1, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_score(const GridView& g, double d1, double d2, int kind, const float* d_pts, uint32_t n,
+                        const double* d_poses, int m, double* d_partials, int blocks_per_pose, double* d_out,
+                        hipStream_t st) {
+  if (kind == kScorePerPoint) {
+    hipLaunchKernelGGL(score_kernel, dim3(blocks_per_pose, 1), dim3(kBlock), 0, st, g, d1, d2, kind, d_pts, n, d_poses,
+                       d_partials, d_out);
+    return hipGetLastError();
+  }
+  double* sums = d_partials + (size_t)m * blocks_per_pose * kPartialStride;
+  hipLaunchKernelGGL(score_kernel, dim3(blocks_per_pose, m), dim3(kBlock), 0, st, g, d1, d2, kind, d_pts, n, d_poses,
+                     d_partials, nullptr);
+  hipLaunchKernelGGL(reduce_partials, dim3(1, m), dim3(64), 0, st, d_partials, blocks_per_pose, 2, sums);
+  hipLaunchKernelGGL(finish_scores, dim3((m + 255) / 256), dim3(256), 0, st, sums, m, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_lookup(const GridView& g, int mode, const float* d_q, uint32_t n, uint32_t max_k, uint32_t* d_counts,
+                         int32_t* d_idx, hipStream_t st) {
+  hipLaunchKernelGGL(lookup_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, g, mode, d_q, n, max_k,
+                     d_counts, d_idx);
+  return hipGetLastError();
+}
+
+}  // namespace ndt

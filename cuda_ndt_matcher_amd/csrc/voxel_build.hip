@@ -1,0 +1,659 @@
+// Voxel-grid build on gfx950: the GPU form of set_target's
+// build_voxel_grid_cpu (src/ndt_cuda/src/voxel_grid/cpu.rs:51-75,
+// voxel_grid/types.rs:54-212).
+//
+// Pipeline (all device work on the handle's stream):
+//   1. key_bounds     : per-point voxel key floor(p/res) (f32 division, like
+//                       VoxelCoord::from_point); min/max over the keys.
+//   2. keygen_hist    : compact Morton code of the key relative to the box
+//                       (only the significant bits of each axis), written as
+//                       u32 (<= 32 bits) or u64; per-pass digit histograms.
+//   3. onesweep_pass  : stable LSD radix sort, one kernel per 8-bit digit:
+//                       wave64 ballot ranking, LDS staging, decoupled
+//                       look-back across tiles (dynamic tile ids).
+//   4. segments       : head flags, tile counts, scan, segment starts.
+//   5. voxel_stats    : one thread per segment: shifted f64 moments, mean,
+//                       covariance, 3x3 Jacobi eigen-regularisation, inverse.
+//   6. hash_insert    : open-addressing insert of every valid voxel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "ndt_device.hpp"
+#include "ndt_internal.hpp"
+
+namespace ndt {
+namespace {
+
+constexpr int kBuildBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBuildBlock * kSortItems;  // 4096 keys per tile
+constexpr int kRadix = 256;
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagInc = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1;
+
+__global__ void fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Ordered-int encoding so that atomicMin/atomicMax on int32 keys work.
+__global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restrict__ pts, size_t n, float res,
+                                                          int32_t* bounds /* minx,miny,minz,maxx,maxy,maxz */) {
+  int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    for (int a = 0; a < 3; ++a) {
+      const int32_t k = voxel_key(pts[3 * i + a], res);
+      mn[a] = min(mn[a], k);
+      mx[a] = max(mx[a], k);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      mn[a] = min(mn[a], __shfl_xor(mn[a], off, kWave));
+      mx[a] = max(mx[a], __shfl_xor(mx[a], off, kWave));
+    }
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&bounds[a], mn[a]);
+      atomicMax(&bounds[3 + a], mx[a]);
+    }
+  }
+}
+
+// Interleaves the bits of (rx, ry, rz) with widths (bx, by, bz): bit i of each
+// axis in x, y, z order, skipping axes that have run out of bits.
+__device__ inline uint64_t compact_morton(uint32_t rx, uint32_t ry, uint32_t rz, int bx, int by, int bz) {
+  uint64_t out = 0;
+  int pos = 0;
+  const int nb = max(bx, max(by, bz));
+  for (int i = 0; i < nb; ++i) {
+    if (i < bx) out |= (uint64_t)((rx >> i) & 1u) << pos++;
+    if (i < by) out |= (uint64_t)((ry >> i) & 1u) << pos++;
+    if (i < bz) out |= (uint64_t)((rz >> i) & 1u) << pos++;
+  }
+  return out;
+}
+
+struct KeyParams {
+  float res;
+  int32_t kmin[3];
+  int bits[3];
+  int passes;
+};
+
+template <class K>
+__global__ __launch_bounds__(kBuildBlock) void keygen_hist(const float* __restrict__ pts, size_t n, KeyParams kp,
+                                                           K* __restrict__ keys, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_hist[8 * kRadix];
+  for (int i = threadIdx.x; i < kp.passes * kRadix; i += kBuildBlock) s_hist[i] = 0;
+  __syncthreads();
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t rx = (uint32_t)(voxel_key(pts[3 * i + 0], kp.res) - kp.kmin[0]);
+    const uint32_t ry = (uint32_t)(voxel_key(pts[3 * i + 1], kp.res) - kp.kmin[1]);
+    const uint32_t rz = (uint32_t)(voxel_key(pts[3 * i + 2], kp.res) - kp.kmin[2]);
+    const uint64_t m = compact_morton(rx, ry, rz, kp.bits[0], kp.bits[1], kp.bits[2]);
+    keys[i] = (K)m;
+    for (int p = 0; p < kp.passes; ++p) atomicAdd(&s_hist[p * kRadix + ((m >> (8 * p)) & 0xFF)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kp.passes * kRadix; i += kBuildBlock)
+    if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
+}
+
+// Exclusive scan of each pass's 256-bin histogram (one workgroup).
+__global__ __launch_bounds__(kRadix) void scan_hist(uint32_t* hist, int passes) {
+  __shared__ uint32_t s[kRadix];
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t v = hist[p * kRadix + threadIdx.x];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kRadix; off <<= 1) {
+      const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    hist[p * kRadix + threadIdx.x] = s[threadIdx.x] - v;
+    __syncthreads();
+  }
+}
+
+// One stable LSD pass over the 8-bit digit at `shift`.
+//
+// Keys are loaded wave-striped: item i of lane l in wave w is element
+// tile*4096 + w*1024 + i*64 + l, so (wave, item, lane) order is input order
+// and the ranks below are stable.  Within a wave the rank of an item is the
+// wave's running count of its digit plus the number of lower lanes holding
+// the same digit in this round (found with 8 ballots).  Tile totals per digit
+// are published with a two-bit flag; each digit's thread looks back over
+// earlier tiles until it meets an inclusive prefix.  Tile ids come from an
+// atomic counter, so every tile a workgroup waits on belongs to a workgroup
+// that started earlier and is running.  Keys are staged in LDS in digit order
+// and then written out in runs.
+template <class K>
+__global__ __launch_bounds__(kBuildBlock) void onesweep_pass(const K* __restrict__ keys_in,
+                                                             const uint32_t* __restrict__ vals_in,
+                                                             K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                                             uint32_t n, int shift,
+                                                             const uint32_t* __restrict__ digit_offset,
+                                                             uint32_t* status, uint32_t* tile_counter) {
+  constexpr int kWaves = kBuildBlock / kWave;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wave_hist[kWaves][kRadix];
+  __shared__ uint32_t s_local[kRadix];
+  __shared__ uint32_t s_base[kRadix];
+  __shared__ uint32_t s_scan[kRadix];
+  __shared__ K s_keys[kSortTile];
+  __shared__ uint32_t s_vals[kSortTile];
+
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+  for (int i = tid; i < kWaves * kRadix; i += kBuildBlock) (&s_wave_hist[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t base = tile * kSortTile;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  K key[kSortItems];
+  uint32_t val[kSortItems];
+  uint32_t rank[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = base + wave * (kSortItems * kWave) + i * kWave + lane;
+    const bool valid = idx < n;
+    key[i] = valid ? keys_in[idx] : (K)0;
+    val[i] = valid ? (vals_in ? vals_in[idx] : idx) : 0u;
+    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bal = __ballot((d >> b) & 1u);
+      match &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t lower = __popcll(match & lt_mask);
+    uint32_t before = 0;
+    if (valid) before = s_wave_hist[wave][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && lower == 0) s_wave_hist[wave][d] = before + (uint32_t)__popcll(match);
+    __builtin_amdgcn_wave_barrier();
+    rank[i] = valid ? before + lower : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+
+  // Thread t owns digit t: per-wave exclusive offsets and the tile total.
+  {
+    const int d = tid;
+    uint32_t total = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = s_wave_hist[w][d];
+      s_wave_hist[w][d] = total;
+      total += c;
+    }
+    uint32_t* my_status = status + (size_t)tile * kRadix + d;
+    if (tile == 0) {
+      __hip_atomic_store(my_status, kFlagInc | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_base[d] = digit_offset[d];
+    } else {
+      __hip_atomic_store(my_status, kFlagAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t excl = 0;
+      int64_t j = (int64_t)tile - 1;
+      while (j >= 0) {
+        const uint32_t s =
+            __hip_atomic_load(status + (size_t)j * kRadix + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t flag = s & ~kValueMask;
+        if (flag == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += s & kValueMask;
+        if (flag == kFlagInc) break;
+        --j;
+      }
+      __hip_atomic_store(my_status, kFlagInc | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_base[d] = digit_offset[d] + excl;
+    }
+    s_scan[d] = total;
+  }
+  __syncthreads();
+  // Exclusive scan of the tile totals over digits (Hillis-Steele in LDS).
+  {
+    const uint32_t own = s_scan[tid];
+    for (int off = 1; off < kRadix; off <<= 1) {
+      const uint32_t t = tid >= off ? s_scan[tid - off] : 0u;
+      __syncthreads();
+      s_scan[tid] += t;
+      __syncthreads();
+    }
+    s_local[tid] = s_scan[tid] - own;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    if (rank[i] == 0xFFFFFFFFu) continue;
+    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
+    const uint32_t pos = s_local[d] + s_wave_hist[wave][d] + rank[i];
+    s_keys[pos] = key[i];
+    s_vals[pos] = val[i];
+  }
+  __syncthreads();
+  const uint32_t count = n - base < (uint32_t)kSortTile ? n - base : (uint32_t)kSortTile;
+  for (uint32_t j = tid; j < count; j += kBuildBlock) {
+    const K k = s_keys[j];
+    const uint32_t d = (uint32_t)(k >> shift) & 0xFFu;
+    const uint32_t g = s_base[d] + (j - s_local[d]);
+    keys_out[g] = k;
+    vals_out[g] = s_vals[j];
+  }
+}
+
+// Segment heads: flag[i] = (i == 0 || key[i] != key[i-1]).  Three phases:
+// per-tile head counts, a single-workgroup scan over tiles, and the write of
+// segment starts.
+template <class K>
+__global__ __launch_bounds__(kBuildBlock) void seg_count(const K* __restrict__ keys, uint32_t n,
+                                                         uint32_t* __restrict__ tile_heads) {
+  __shared__ uint32_t s[kBuildBlock / kWave];
+  const uint32_t base = blockIdx.x * kSortTile;
+  uint32_t c = 0;
+  for (uint32_t j = threadIdx.x; j < kSortTile; j += kBuildBlock) {
+    const uint32_t i = base + j;
+    if (i < n && (i == 0 || keys[i] != keys[i - 1])) ++c;
+  }
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) s[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kBuildBlock / kWave; ++w) t += s[w];
+    tile_heads[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void seg_scan_tiles(uint32_t* tile_heads, uint32_t tiles, uint32_t* total) {
+  __shared__ uint32_t s[1024];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < tiles; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < tiles ? tile_heads[i] : 0u;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const uint32_t t = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < tiles) tile_heads[i] = carry + s[threadIdx.x] - v;
+    const uint32_t block_total = s[1023];
+    __syncthreads();
+    carry += block_total;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+template <class K>
+__global__ __launch_bounds__(kBuildBlock) void seg_write(const K* __restrict__ keys, uint32_t n,
+                                                         const uint32_t* __restrict__ tile_offset,
+                                                         uint32_t* __restrict__ seg_start) {
+  // Each thread owns 16 consecutive elements of the tile; a block scan of the
+  // per-thread head counts gives each head its segment id.
+  __shared__ uint32_t s[kBuildBlock];
+  const uint32_t base = blockIdx.x * kSortTile + threadIdx.x * kSortItems;
+  uint32_t c = 0;
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t i = base + k;
+    if (i < n && (i == 0 || keys[i] != keys[i - 1])) ++c;
+  }
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 1; off < kBuildBlock; off <<= 1) {
+    const uint32_t t = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t seg = tile_offset[blockIdx.x] + s[threadIdx.x] - c;
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t i = base + k;
+    if (i < n && (i == 0 || keys[i] != keys[i - 1])) seg_start[seg++] = i;
+  }
+}
+
+struct StatsParams {
+  float res;
+  uint32_t min_points;
+  double eig_ratio;  // (double)0.01f
+  uint32_t segments;
+  uint32_t n;
+};
+
+// Voxel::from_statistics + regularize_covariance (voxel_grid/types.rs:54-190)
+// for one segment.  Moments are accumulated in f64 about the segment's first
+// point (shifted data), which is exact where the reference's uncentered sums
+// lose digits; mean and covariance agree with the reference formula to
+// rounding.
+__global__ __launch_bounds__(kBuildBlock) void voxel_stats(const float* __restrict__ pts,
+                                                           const uint32_t* __restrict__ order,
+                                                           const uint32_t* __restrict__ seg_start, StatsParams sp,
+                                                           float4* __restrict__ vox, float* __restrict__ cov_out,
+                                                           float* __restrict__ axis_out, uint32_t* __restrict__ count_out,
+                                                           int32_t* __restrict__ coord_out, uint32_t* __restrict__ valid) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= sp.segments) return;
+  const uint32_t b = seg_start[s];
+  const uint32_t e = (s + 1 < sp.segments) ? seg_start[s + 1] : sp.n;
+  const uint32_t cnt = e - b;
+  const uint32_t first = order[b];
+  const float p0x = pts[3 * first], p0y = pts[3 * first + 1], p0z = pts[3 * first + 2];
+  coord_out[3 * s + 0] = voxel_key(p0x, sp.res);
+  coord_out[3 * s + 1] = voxel_key(p0y, sp.res);
+  coord_out[3 * s + 2] = voxel_key(p0z, sp.res);
+  count_out[s] = cnt;
+  valid[s] = 0;
+  if (cnt < sp.min_points) return;
+  const double ox = p0x, oy = p0y, oz = p0z;
+  double sx = 0, sy = 0, sz = 0, sxx = 0, sxy = 0, sxz = 0, syy = 0, syz = 0, szz = 0;
+  for (uint32_t j = b; j < e; ++j) {
+    const uint32_t i = order[j];
+    const double x = (double)pts[3 * i] - ox, y = (double)pts[3 * i + 1] - oy, z = (double)pts[3 * i + 2] - oz;
+    sx += x; sy += y; sz += z;
+    sxx += x * x; sxy += x * y; sxz += x * z;
+    syy += y * y; syz += y * z; szz += z * z;
+  }
+  const double n = (double)cnt;
+  const double mx = sx / n, my = sy / n, mz = sz / n;
+  const double inv_n1 = 1.0 / (n - 1.0);
+  double c[9];
+  c[0] = (sxx - sx * mx) * inv_n1;
+  c[4] = (syy - sy * my) * inv_n1;
+  c[8] = (szz - sz * mz) * inv_n1;
+  c[1] = c[3] = (sxy - sx * my) * inv_n1;
+  c[2] = c[6] = (sxz - sx * mz) * inv_n1;
+  c[5] = c[7] = (syz - sy * mz) * inv_n1;
+  double ev[3], V[9];
+  sym_eigen<3>(c, ev, V);
+  const double max_ev = fmax(fmax(fmax(0.0, ev[0]), ev[1]), ev[2]);
+  if (!(max_ev > 0.0)) return;  // types.rs:126-128 (NaN statistics are dropped too)
+  int min_idx = 0;
+  for (int i = 1; i < 3; ++i)
+    if (ev[i] < ev[min_idx]) min_idx = i;
+  const double thr = max_ev * sp.eig_ratio;
+  bool needs = false;
+  double cl[3];
+  for (int i = 0; i < 3; ++i) {
+    needs |= ev[i] < thr;
+    cl[i] = ev[i] < thr ? thr : ev[i];
+  }
+  double reg[9], inv[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double r = 0.0, q = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        r += V[i * 3 + k] * cl[k] * V[j * 3 + k];
+        q += V[i * 3 + k] * (1.0 / cl[k]) * V[j * 3 + k];
+      }
+      reg[i * 3 + j] = needs ? r : c[i * 3 + j];
+      inv[i * 3 + j] = q;
+    }
+  const float fmx = (float)(ox + mx), fmy = (float)(oy + my), fmz = (float)(oz + mz);
+  vox[3 * s + 0] = make_float4(fmx, fmy, fmz, (float)inv[0]);
+  vox[3 * s + 1] = make_float4((float)inv[1], (float)inv[2], (float)inv[3], (float)inv[4]);
+  vox[3 * s + 2] = make_float4((float)inv[5], (float)inv[6], (float)inv[7], (float)inv[8]);
+  for (int i = 0; i < 9; ++i) cov_out[9 * s + i] = (float)reg[i];
+  for (int i = 0; i < 3; ++i) axis_out[3 * s + i] = (float)V[i * 3 + min_idx];
+  valid[s] = 1;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void hash_insert(const int32_t* __restrict__ coords,
+                                                           const uint32_t* __restrict__ valid, uint32_t segments,
+                                                           uint4* table, uint32_t mask) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= segments || !valid[s]) return;
+  const uint64_t key = pack_coord(coords[3 * s], coords[3 * s + 1], coords[3 * s + 2]);
+  uint32_t h = (uint32_t)mix64(key) & mask;
+  for (;;) {
+    unsigned long long* slot = reinterpret_cast<unsigned long long*>(&table[h]);
+    const unsigned long long prev = atomicCAS(slot, (unsigned long long)kEmptyKey, (unsigned long long)key);
+    if (prev == kEmptyKey || prev == key) {
+      table[h].z = s;
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// Bounding box of the valid voxel coordinates (for the lookup's fast reject).
+__global__ __launch_bounds__(kBuildBlock) void valid_bounds(const int32_t* __restrict__ coords,
+                                                            const uint32_t* __restrict__ valid, uint32_t segments,
+                                                            int32_t* bounds, uint32_t* nvalid) {
+  int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  uint32_t c = 0;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < segments; s += gridDim.x * blockDim.x) {
+    if (!valid[s]) continue;
+    ++c;
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = min(mn[a], coords[3 * s + a]);
+      mx[a] = max(mx[a], coords[3 * s + a]);
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    c += __shfl_xor(c, off, kWave);
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = min(mn[a], __shfl_xor(mn[a], off, kWave));
+      mx[a] = max(mx[a], __shfl_xor(mx[a], off, kWave));
+    }
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicAdd(nvalid, c);
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&bounds[a], mn[a]);
+      atomicMax(&bounds[3 + a], mx[a]);
+    }
+  }
+}
+
+__global__ void init_bounds(int32_t* b, uint32_t* extra, int nextra) {
+  if (threadIdx.x < 3) b[threadIdx.x] = INT32_MAX;
+  else if (threadIdx.x < 6) b[threadIdx.x] = INT32_MIN;
+  if ((int)threadIdx.x < nextra) extra[threadIdx.x] = 0;
+}
+
+int grid_for(size_t n, int block, int max_blocks) {
+  size_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > (size_t)max_blocks) g = max_blocks;
+  return (int)g;
+}
+
+template <class K>
+hipError_t run_sort(DeviceGrid& G, const KeyParams& kp, size_t n, hipStream_t st) {
+  const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  K* keys_a = reinterpret_cast<K*>(G.sort_keys_a.ptr);
+  K* keys_b = reinterpret_cast<K*>(G.sort_keys_b.ptr);
+  uint32_t* vals_a = reinterpret_cast<uint32_t*>(G.sort_vals_a.ptr);
+  uint32_t* vals_b = reinterpret_cast<uint32_t*>(G.sort_vals_b.ptr);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(G.hist.ptr);
+  uint32_t* status = reinterpret_cast<uint32_t*>(G.status.ptr);
+  uint32_t* counters = hist + 8 * kRadix;
+  const size_t status_words = (size_t)kp.passes * tiles * kRadix;
+  hipLaunchKernelGGL(fill_u32, dim3(grid_for(status_words, 256, 2048)), dim3(256), 0, st, status, status_words, 0u);
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, hist, (size_t)(8 * kRadix + 8), 0u);
+  hipLaunchKernelGGL(keygen_hist<K>, dim3(grid_for(n, kBuildBlock, 1024)), dim3(kBuildBlock), 0, st, G.points, n, kp,
+                     keys_a, hist);
+  hipLaunchKernelGGL(scan_hist, dim3(1), dim3(kRadix), 0, st, hist, kp.passes);
+  (void)hipEventRecord(G.ev_sort0, st);
+  const K* kin = keys_a;
+  const uint32_t* vin = nullptr;
+  K* kout = keys_b;
+  uint32_t* vout = vals_b;
+  for (int p = 0; p < kp.passes; ++p) {
+    hipLaunchKernelGGL(onesweep_pass<K>, dim3(tiles), dim3(kBuildBlock), 0, st, kin, vin, kout, vout, (uint32_t)n,
+                       8 * p, hist + p * kRadix, status + (size_t)p * tiles * kRadix, counters + p);
+    // ping-pong
+    const K* nk = kout;
+    const uint32_t* nv = vout;
+    kout = (kout == keys_b) ? keys_a : keys_b;
+    vout = (vout == vals_b) ? vals_a : vals_b;
+    kin = nk;
+    vin = nv;
+  }
+  (void)hipEventRecord(G.ev_sort1, st);
+  G.sorted_keys = const_cast<K*>(kin);
+  G.sorted_vals = const_cast<uint32_t*>(vin);
+  return hipGetLastError();
+}
+
+template <class K>
+hipError_t run_segments(DeviceGrid& G, size_t n, hipStream_t st, uint32_t* segments_out) {
+  const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  const K* keys = reinterpret_cast<const K*>(G.sorted_keys);
+  uint32_t* tile_heads = reinterpret_cast<uint32_t*>(G.tile_heads.ptr);
+  uint32_t* scalars = reinterpret_cast<uint32_t*>(G.scalars.ptr);
+  hipLaunchKernelGGL(seg_count<K>, dim3(tiles), dim3(kBuildBlock), 0, st, keys, (uint32_t)n, tile_heads);
+  hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_heads, tiles, scalars);
+  hipLaunchKernelGGL(seg_write<K>, dim3(tiles), dim3(kBuildBlock), 0, st, keys, (uint32_t)n, tile_heads,
+                     reinterpret_cast<uint32_t*>(G.seg_start.ptr));
+  hipError_t e = hipMemcpyAsync(G.host_scalars, scalars, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  *segments_out = G.host_scalars[0];
+  return hipGetLastError();
+}
+
+uint32_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return (uint32_t)p;
+}
+
+}  // namespace
+
+// Builds the grid from G.points (device, n points).  Returns a status code and
+// fills G's voxel arrays, hash and views.
+int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStream_t st, std::string* err) {
+  auto fail = [&](hipError_t e, const char* what) {
+    *err = std::string(what) + ": " + hipGetErrorString(e);
+    return -5;
+  };
+  if (n == 0) {
+    *err = "Target point cloud is empty";
+    return -2;
+  }
+  if (n >= (1ull << 30)) {
+    *err = "target exceeds 2^30 points";
+    return -7;
+  }
+  hipError_t e;
+  (void)hipEventRecord(G.ev_build0, st);
+  if ((e = G.scalars.reserve(64)) != hipSuccess) return fail(e, "alloc");
+  int32_t* bounds = reinterpret_cast<int32_t*>(G.scalars.ptr) + 8;
+  uint32_t* extra = reinterpret_cast<uint32_t*>(G.scalars.ptr);
+  hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra, 8);
+  hipLaunchKernelGGL(key_bounds, dim3(grid_for(n, kBuildBlock, 1024)), dim3(kBuildBlock), 0, st, G.points, n, res,
+                     bounds);
+  if ((e = hipMemcpyAsync(G.host_scalars + 8, bounds, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return fail(e, "bounds copy");
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "bounds sync");
+  int32_t kb[6];
+  std::memcpy(kb, G.host_scalars + 8, sizeof(kb));
+  KeyParams kp;
+  kp.res = res;
+  int total_bits = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t span = (int64_t)kb[3 + a] - (int64_t)kb[a];
+    if (kb[a] < -kCoordBias || kb[3 + a] >= kCoordBias || span >= (1ll << 21)) {
+      *err = "map extent exceeds 2^20 voxels from the origin on some axis";
+      return -7;
+    }
+    int bits = 0;
+    while (((int64_t)1 << bits) <= span) ++bits;
+    kp.kmin[a] = kb[a];
+    kp.bits[a] = bits;
+    total_bits += bits;
+  }
+  kp.passes = std::max(1, (total_bits + 7) / 8);
+  G.key_bits = total_bits;
+  G.key64 = total_bits > 32;
+  std::memcpy(G.key_min, kp.kmin, sizeof(kp.kmin));
+  std::memcpy(G.key_bits_axis, kp.bits, sizeof(kp.bits));
+  const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  const size_t ksz = G.key64 ? 8 : 4;
+  if ((e = G.sort_keys_a.reserve(n * ksz)) != hipSuccess) return fail(e, "alloc keys");
+  if ((e = G.sort_keys_b.reserve(n * ksz)) != hipSuccess) return fail(e, "alloc keys");
+  if ((e = G.sort_vals_a.reserve(n * 4)) != hipSuccess) return fail(e, "alloc vals");
+  if ((e = G.sort_vals_b.reserve(n * 4)) != hipSuccess) return fail(e, "alloc vals");
+  if ((e = G.hist.reserve((8 * kRadix + 8) * 4)) != hipSuccess) return fail(e, "alloc hist");
+  if ((e = G.status.reserve((size_t)kp.passes * tiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc status");
+  if ((e = G.tile_heads.reserve((size_t)tiles * 4)) != hipSuccess) return fail(e, "alloc tiles");
+  if ((e = G.seg_start.reserve(n * 4)) != hipSuccess) return fail(e, "alloc segs");
+  e = G.key64 ? run_sort<uint64_t>(G, kp, n, st) : run_sort<uint32_t>(G, kp, n, st);
+  if (e != hipSuccess) return fail(e, "radix sort");
+  uint32_t segs = 0;
+  e = G.key64 ? run_segments<uint64_t>(G, n, st, &segs) : run_segments<uint32_t>(G, n, st, &segs);
+  if (e != hipSuccess) return fail(e, "segments");
+  G.segments = segs;
+  if ((e = G.vox.reserve((size_t)segs * 48)) != hipSuccess) return fail(e, "alloc voxels");
+  if ((e = G.cov.reserve((size_t)segs * 36)) != hipSuccess) return fail(e, "alloc voxels");
+  if ((e = G.axis.reserve((size_t)segs * 12)) != hipSuccess) return fail(e, "alloc voxels");
+  if ((e = G.count.reserve((size_t)segs * 4)) != hipSuccess) return fail(e, "alloc voxels");
+  if ((e = G.coord.reserve((size_t)segs * 12)) != hipSuccess) return fail(e, "alloc voxels");
+  if ((e = G.valid.reserve((size_t)segs * 4)) != hipSuccess) return fail(e, "alloc voxels");
+  StatsParams sp;
+  sp.res = res;
+  sp.min_points = min_points;
+  sp.eig_ratio = (double)0.01f;  // ndt.rs:369 passes 0.01 as f32 (voxel_grid/types.rs:138)
+  sp.segments = segs;
+  sp.n = (uint32_t)n;
+  hipLaunchKernelGGL(voxel_stats, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st, G.points,
+                     G.sorted_vals, reinterpret_cast<const uint32_t*>(G.seg_start.ptr), sp,
+                     reinterpret_cast<float4*>(G.vox.ptr), reinterpret_cast<float*>(G.cov.ptr),
+                     reinterpret_cast<float*>(G.axis.ptr), reinterpret_cast<uint32_t*>(G.count.ptr),
+                     reinterpret_cast<int32_t*>(G.coord.ptr), reinterpret_cast<uint32_t*>(G.valid.ptr));
+  const uint32_t cap = std::max<uint32_t>(1024, next_pow2(2ull * segs));
+  if ((e = G.table.reserve((size_t)cap * 16)) != hipSuccess) return fail(e, "alloc hash");
+  hipLaunchKernelGGL(fill_u32, dim3(grid_for((size_t)cap * 4, 256, 4096)), dim3(256), 0, st,
+                     reinterpret_cast<uint32_t*>(G.table.ptr), (size_t)cap * 4, 0xFFFFFFFFu);
+  hipLaunchKernelGGL(hash_insert, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+                     reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
+                     segs, reinterpret_cast<uint4*>(G.table.ptr), cap - 1);
+  hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra + 1, 1);
+  hipLaunchKernelGGL(valid_bounds, dim3(grid_for(segs, kBuildBlock, 512)), dim3(kBuildBlock), 0, st,
+                     reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
+                     segs, bounds, extra + 1);
+  if ((e = hipMemcpyAsync(G.host_scalars, extra, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return fail(e, "voxel count copy");
+  (void)hipEventRecord(G.ev_build1, st);
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
+  G.num_valid = G.host_scalars[1];
+  const int32_t* vb = reinterpret_cast<const int32_t*>(G.host_scalars + 8);
+  GridView& v = G.view;
+  v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
+  v.axis = reinterpret_cast<const float*>(G.axis.ptr);
+  v.table = reinterpret_cast<const uint4*>(G.table.ptr);
+  v.table_mask = cap - 1;
+  for (int a = 0; a < 3; ++a) {
+    v.cmin[a] = vb[a];
+    v.cmax[a] = vb[3 + a];
+  }
+  v.res = res;
+  G.n_points = n;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, G.ev_build0, G.ev_build1);
+  G.build_ms = ms;
+  (void)hipEventElapsedTime(&ms, G.ev_sort0, G.ev_sort1);
+  G.sort_ms = ms;
+  if (G.num_valid == 0) {
+    *err = "No voxels created from target points (too sparse?)";
+    return -4;
+  }
+  return 0;
+}
+
+}  // namespace ndt

@@ -1,0 +1,203 @@
+/*
+ * ndt_hip.h - C ABI of the MI355X NDT scan-matching library (libndt_hip.so).
+ *
+ * The library replaces the scan-matching hot path of the reference crate
+ * `ndt_cuda` (jerry73204/cuda_ndt_matcher, src/ndt_cuda).  Each entry point
+ * below names the reference interface it stands in for (file:line, relative
+ * to the reference repository root).  Semantics follow the reference's CPU
+ * path (NDT_USE_GPU=0), which is the one the node's results are defined by.
+ *
+ * Conventions
+ *  - Every function returns an int: 0 on success, a negative ndt_status
+ *    otherwise; ndt_hip_last_error(h) gives a message for the last failure.
+ *  - Points are AoS float triples [[x,y,z],...] as in the reference's
+ *    `&[[f32; 3]]`.  Host arrays are borrowed for the duration of the call.
+ *    `*_device` variants take device pointers (HBM-resident inputs).
+ *  - A pose is the internal 6-vector [tx,ty,tz,roll,pitch,yaw] (f64) that the
+ *    reference produces with isometry_to_pose_vector
+ *    (src/ndt_cuda/src/optimization/types.rs:164-174, nalgebra euler_angles)
+ *    and turns back into an Isometry3 with pose_vector_to_isometry
+ *    (types.rs:156-161).  Callers holding an Isometry3 apply those two
+ *    conversions on their side of the boundary.
+ *  - A handle owns one HIP stream and all of its device memory.  A handle is
+ *    not thread-safe; different handles may be used concurrently.
+ */
+#ifndef NDT_HIP_H
+#define NDT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes.  Mirrors the reference's error classes: anyhow bails in
+ * ndt.rs (empty cloud, no target, zero voxels) and CudaError from cuda_ffi
+ * (src/cuda_ffi/src/radix_sort.rs:59-94). */
+typedef enum ndt_status {
+  NDT_OK = 0,
+  NDT_ERR_INVALID_ARGUMENT = -1, /* null pointer, bad enum, bad size */
+  NDT_ERR_EMPTY_INPUT = -2,      /* "Target point cloud is empty" / "Source point cloud is empty" */
+  NDT_ERR_NO_TARGET = -3,        /* "No target set. Call set_target() first." */
+  NDT_ERR_NO_VOXELS = -4,        /* "No voxels created from target points (too sparse?)" */
+  NDT_ERR_HIP = -5,              /* a HIP runtime call failed */
+  NDT_ERR_UNSUPPORTED = -6,      /* a configuration this build does not implement */
+  NDT_ERR_CAPACITY = -7          /* input exceeds a documented limit */
+} ndt_status;
+
+/* Neighbour search for the derivative, NVTL and TP passes. */
+typedef enum ndt_search_mode {
+  NDT_SEARCH_KDTREE = 0,   /* every voxel whose mean is within `resolution` (the reference's
+                              KD-tree radius search, voxel_grid/search.rs:90-99) */
+  NDT_SEARCH_DIRECT26 = 1, /* the 26 cells around the point's cell (Autoware pclomp DIRECT26) */
+  NDT_SEARCH_DIRECT7 = 2,  /* the point's cell and its 6 face neighbours */
+  NDT_SEARCH_DIRECT1 = 3   /* the point's cell */
+} ndt_search_mode;
+
+typedef enum ndt_distance_metric {
+  NDT_METRIC_POINT_TO_DISTRIBUTION = 0, /* derivatives/types.rs:10-25 */
+  NDT_METRIC_POINT_TO_PLANE = 1
+} ndt_distance_metric;
+
+typedef enum ndt_precision {
+  NDT_PRECISION_FP64 = 0,    /* per-point terms and accumulation in f64 */
+  NDT_PRECISION_FP32 = 1     /* per-point terms in f32 with compensated (two-sum) f64-free
+                                accumulation; f64 for the 6x6 solve and the pose */
+} ndt_precision;
+
+/* NdtScanMatcherConfig (src/ndt_cuda/src/ndt.rs:41-107) plus the build's own
+ * knobs.  ndt_hip_config_default() fills the reference defaults. */
+typedef struct ndt_config {
+  float resolution;                   /* 2.0 */
+  uint32_t max_iterations;            /* 30 */
+  double trans_epsilon;               /* 0.01 */
+  double step_size;                   /* 0.1 */
+  double outlier_ratio;               /* 0.55 */
+  double regularization;              /* 0.001 (kept for API parity; the CPU path ignores it) */
+  int32_t use_line_search;            /* 0 */
+  uint32_t min_points_per_voxel;      /* 6 */
+  int32_t regularization_enabled;     /* 0: GNSS regularisation term (optimization/regularization.rs) */
+  double regularization_scale_factor; /* 0.01 */
+  int32_t distance_metric;            /* ndt_distance_metric */
+  int32_t search_mode;                /* ndt_search_mode, default NDT_SEARCH_KDTREE */
+  int32_t precision;                  /* ndt_precision, default NDT_PRECISION_FP64 */
+  int32_t workgroups_per_alignment;   /* 0 = automatic; otherwise workgroups cooperating on one alignment */
+} ndt_config;
+
+/* Convergence status (optimization/types.rs:66-82). */
+typedef enum ndt_convergence {
+  NDT_CONVERGED = 0,
+  NDT_MAX_ITERATIONS = 1,
+  NDT_NO_CORRESPONDENCES = 2,
+  NDT_SINGULAR_HESSIAN = 3
+} ndt_convergence;
+
+/* AlignResult (ndt.rs:220-248) with NdtResult's status (optimization/types.rs:100-131). */
+typedef struct ndt_result {
+  double pose[6];
+  int32_t status;                /* ndt_convergence */
+  int32_t converged;             /* status == NDT_CONVERGED */
+  double score;
+  double transform_probability;  /* score / number of source points (solver.rs:707-713) */
+  double nvtl;
+  uint32_t iterations;
+  uint32_t num_correspondences;
+  uint32_t oscillation_count;
+  uint32_t reserved;
+  double hessian[36];            /* row-major */
+} ndt_result;
+
+/* Device timing of the last call on a handle, from HIP events recorded on the
+ * handle's stream around each kernel group. */
+typedef struct ndt_timing {
+  float build_ms;     /* last set_target: whole voxel-grid build */
+  float sort_ms;      /* part of build_ms spent in the radix sort passes */
+  float align_ms;     /* last align*: the persistent Newton kernel */
+  float score_ms;     /* last evaluate_*: the scoring kernel */
+  uint32_t align_workgroups;
+  uint32_t align_slots;
+} ndt_timing;
+
+typedef struct ndt_handle ndt_handle;
+
+void ndt_hip_config_default(ndt_config* cfg);
+
+/* NdtScanMatcher::with_config / builder().build() (ndt.rs:287-330, 200-210). */
+int ndt_hip_create(const ndt_config* cfg, ndt_handle** out);
+void ndt_hip_destroy(ndt_handle* h);
+const char* ndt_hip_last_error(const ndt_handle* h);
+
+/* NdtScanMatcher::set_target (ndt.rs:361-415).  Builds the voxel grid on the
+ * GPU (Morton keys, LDS onesweep radix sort, segmented moments, eigen
+ * regularisation, voxel hash).  Replaces the CUB bindings of cuda_ffi
+ * (radix_sort.cu:24,62; segment_detect.cu; voxel_hash.cu:218-386). */
+int ndt_hip_set_target(ndt_handle* h, const float* xyz, size_t n);
+int ndt_hip_set_target_device(ndt_handle* h, const float* d_xyz, size_t n);
+/* has_target / target_grid().len() (ndt.rs:343-355). */
+int ndt_hip_has_target(const ndt_handle* h);
+int ndt_hip_num_voxels(const ndt_handle* h, size_t* out);
+
+/* Copies the voxel grid to host arrays (any pointer may be NULL).  Voxels are
+ * ordered by Morton key.  Sizes: coords 3*V, means 3*V, covs 9*V, invs 9*V,
+ * axes 3*V, counts V. */
+int ndt_hip_get_voxels(ndt_handle* h, int32_t* coords, float* means, float* covs, float* invs, float* axes,
+                       uint32_t* counts);
+
+/* Build internals for parity tests: for every map point its voxel key (3*N
+ * int32), the stable sort permutation (N uint32: original index at each sorted
+ * position) and the packed Morton key at each sorted position (N uint64). */
+int ndt_hip_get_build_debug(ndt_handle* h, int32_t* point_keys, uint32_t* sorted_index, uint64_t* sorted_keys);
+
+/* Neighbour lookup for parity tests: for each query point (f32 xyz), up to
+ * max_k voxel indices (into the ndt_hip_get_voxels order) and the full count. */
+int ndt_hip_lookup(ndt_handle* h, int32_t mode, const float* q, size_t n, uint32_t max_k, uint32_t* counts,
+                   int32_t* indices);
+
+/* compute_derivatives_cpu_with_metric (derivatives/cpu.rs:417-538): score,
+ * gradient, Hessian and correspondence count at one pose. */
+int ndt_hip_derivatives(ndt_handle* h, const float* xyz, size_t n, const double pose[6], int32_t compute_hessian,
+                        double* score, double grad[6], double hess[36], uint32_t* num_correspondences);
+
+/* NdtScanMatcher::align (ndt.rs:421-476) -> NdtOptimizer::align (solver.rs:135-305). */
+int ndt_hip_align(ndt_handle* h, const float* xyz, size_t n, const double pose[6], ndt_result* out);
+int ndt_hip_align_device(ndt_handle* h, const float* d_xyz, size_t n, const double pose[6], ndt_result* out);
+
+/* align_batch / align_batch_gpu (ndt.rs:903-1001): one scan, m initial poses,
+ * all in one persistent launch. */
+int ndt_hip_align_batch(ndt_handle* h, const float* xyz, size_t n, const double* poses, size_t m, ndt_result* out);
+int ndt_hip_align_batch_device(ndt_handle* h, const float* d_xyz, size_t n, const double* poses, size_t m,
+                               ndt_result* out);
+
+/* align_parallel_scans (ndt.rs:1224-1320): m scans of uneven size with their
+ * own poses, in one launch.  Unlike the reference's GPU batch (which reports
+ * NVTL 0 and TP = score/#corr, ndt.rs:1305-1310) every result carries the
+ * CPU-path NVTL and TP. */
+int ndt_hip_align_scans(ndt_handle* h, const float* const* xyz, const size_t* n, const double* poses, size_t m,
+                        ndt_result* out);
+
+/* evaluate_nvtl / evaluate_transform_probability (ndt.rs:740-797). */
+int ndt_hip_evaluate_nvtl(ndt_handle* h, const float* xyz, size_t n, const double pose[6], double* out);
+int ndt_hip_evaluate_transform_probability(ndt_handle* h, const float* xyz, size_t n, const double pose[6],
+                                           double* out);
+/* compute_per_point_scores (ndt.rs:800-817, scoring/metrics.rs:137-178). */
+int ndt_hip_per_point_scores(ndt_handle* h, const float* xyz, size_t n, const double pose[6], double* out);
+/* evaluate_nvtl_batch (ndt.rs:833-890): m poses, 3x3x3-cell NVTL (scoring/nvtl.rs:83-190). */
+int ndt_hip_evaluate_nvtl_batch(ndt_handle* h, const float* xyz, size_t n, const double* poses, size_t m,
+                                double* out);
+
+/* set_/clear_regularization_pose (ndt.rs:1151-1160); translation only is used. */
+int ndt_hip_set_regularization_pose(ndt_handle* h, const double pose[6]);
+int ndt_hip_clear_regularization_pose(ndt_handle* h);
+
+/* estimate_xy_covariance_by_laplace (cuda_ndt_matcher/src/covariance.rs:82-98)
+ * and its 6x6 extension -H^-1 (identity when H is singular). */
+int ndt_hip_laplace_covariance_xy(const double hess[36], double cov[4]);
+int ndt_hip_laplace_covariance_6x6(const double hess[36], double cov[36]);
+
+int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,275 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Bar: voxel keys, sort order, point-to-voxel assignment and counts bit-exact;
+per-voxel means and covariances within 1e-5 relative; neighbour sets equal;
+alignment iterations and status exact, pose within 1 mm / 0.01 deg, NVTL and
+TP within 1e-4.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests import npref
+
+pytestmark = pytest.mark.gpu
+
+POSE_MM = 1e-3
+POSE_RAD = np.deg2rad(0.01)
+SCORE_TOL = 1e-4
+
+
+def _print_case(name):
+    print(f"[gpu-case] {name}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def small():
+    from cuda_ndt_matcher_amd import synth
+
+    return synth.make_workload("small", 0)
+
+
+@pytest.fixture(scope="module")
+def oracle_grid(small, oracle_mod):
+    return oracle_mod.Grid(small.map_points, 2.0)
+
+
+def _matcher(points=None, **cfg):
+    from cuda_ndt_matcher_amd import NdtScanMatcher, NdtScanMatcherConfig
+
+    m = NdtScanMatcher(NdtScanMatcherConfig(**cfg))
+    if points is not None:
+        m.set_target(points)
+    return m
+
+
+@pytest.fixture(scope="module")
+def gpu_small(small):
+    _print_case("set_target small")
+    return _matcher(small.map_points)
+
+
+def morton_np(keys):
+    """Compact Morton code of integer keys relative to their bounding box."""
+    kmin = keys.min(0)
+    r = (keys - kmin).astype(np.uint64)
+    span = (keys.max(0) - kmin)
+    bits = [int(s).bit_length() for s in span]
+    out = np.zeros(keys.shape[0], np.uint64)
+    pos = 0
+    for i in range(max(bits)):
+        for a in range(3):
+            if i < bits[a]:
+                out |= ((r[:, a] >> np.uint64(i)) & np.uint64(1)) << np.uint64(pos)
+                pos += 1
+    return out
+
+
+def _check_build(matcher, points, oracle_grid, oracle_mod):
+    n = points.shape[0]
+    dbg = matcher.build_debug(n)
+    keys = oracle_mod.point_keys(points, 2.0)
+    # voxel key of every point, bit-exact
+    assert np.array_equal(dbg["point_keys"], keys)
+    # stable sort order of the packed key, bit-exact
+    mk = morton_np(keys.astype(np.int64))
+    order = np.argsort(mk, kind="stable")
+    assert np.array_equal(dbg["sorted_index"], order.astype(np.uint32))
+    assert np.array_equal(dbg["sorted_keys"], mk[order])
+    # voxels: same set of coordinates, same counts, statistics within 1e-5
+    g = matcher.target_grid()
+    ex = oracle_grid.export()
+    gi = np.lexsort(g["coords"].T[::-1])
+    oi = np.lexsort(ex["coords"].T[::-1])
+    assert np.array_equal(g["coords"][gi], ex["coords"][oi])
+    assert np.array_equal(g["counts"][gi], ex["counts"][oi])
+    for name in ("means", "covs", "invs"):
+        a = g[name][gi].astype(np.float64).reshape(len(gi), -1)
+        b = ex[name][oi].astype(np.float64).reshape(len(oi), -1)
+        scale = np.maximum(np.abs(b).max(1, keepdims=True), 1e-30)
+        err = np.abs(a - b) / scale
+        assert err.max() <= 1e-5, f"{name}: max rel err {err.max():.3e}"
+    # principal axis up to sign
+    dot = np.abs(np.sum(g["axes"][gi].astype(np.float64) * ex["axes"][oi], 1))
+    assert np.percentile(dot, 99) > 1 - 1e-4
+    return gi, oi
+
+
+def test_build_parity_small(gpu_small, small, oracle_grid, oracle_mod):
+    _print_case("build parity small")
+    _check_build(gpu_small, small.map_points, oracle_grid, oracle_mod)
+
+
+def _coord_sets(coords, cnt, idx):
+    out = []
+    for i in range(len(cnt)):
+        out.append(sorted(tuple(coords[j]) for j in idx[i, :cnt[i]]))
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_lookup_parity(gpu_small, small, oracle_grid, mode):
+    _print_case(f"lookup mode {mode}")
+    rng = np.random.default_rng(mode)
+    R = npref.rot_xyz(*small.guess[3:])
+    q = (small.scan.astype(np.float64) @ R.T + small.guess[:3]).astype(np.float32)
+    q = np.concatenate([q, (q[:500] + rng.normal(0, 1.0, (500, 3))).astype(np.float32)])
+    cnt_g, idx_g = gpu_small.lookup(q, mode, max_k=40)
+    cnt_o, idx_o = oracle_grid.search(q, mode, max_k=40)
+    assert np.array_equal(cnt_g, cnt_o)
+    g = gpu_small.target_grid()
+    ex = oracle_grid.export()
+    assert _coord_sets(g["coords"], cnt_g, idx_g) == _coord_sets(ex["coords"], cnt_o, idx_o)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_derivatives_parity(gpu_small, small, oracle_grid, oracle_mod, mode):
+    _print_case(f"derivatives mode {mode}")
+    from cuda_ndt_matcher_amd import NdtScanMatcherConfig  # noqa: F401
+
+    gpu = _matcher(small.map_points, search_mode=mode) if mode else gpu_small
+    rng = np.random.default_rng(3)
+    p = oracle_mod.params(search_mode=mode)
+    for k in range(4):
+        pose = small.guess + np.concatenate([rng.normal(0, 0.2, 3), rng.normal(0, 0.02, 3)])
+        for hess in (True, False):
+            s_g, g_g, H_g, c_g = gpu.derivatives(small.scan, pose, hess)
+            s_o, g_o, H_o, c_o = oracle_grid.derivatives(small.scan, pose, p, hess)
+            assert c_g == c_o
+            assert s_g == pytest.approx(s_o, rel=1e-10)
+            np.testing.assert_allclose(g_g, g_o, rtol=0, atol=1e-9 * np.abs(g_o).max())
+            if hess:
+                np.testing.assert_allclose(H_g, H_o, rtol=0, atol=1e-9 * np.abs(H_o).max())
+
+
+def _compare_align(r, o, n_points):
+    assert r.status == ["Converged", "MaxIterations", "NoCorrespondences", "SingularHessian"][o.status]
+    assert r.iterations == o.iterations
+    assert np.abs(r.pose_vector[:3] - o.pose[:3]).max() <= POSE_MM
+    assert np.abs(r.pose_vector[3:] - o.pose[3:]).max() <= POSE_RAD
+    assert r.nvtl == pytest.approx(o.nvtl, abs=SCORE_TOL)
+    assert r.transform_probability == pytest.approx(o.transform_probability, abs=SCORE_TOL)
+    assert r.num_correspondences == o.num_correspondences
+    assert r.oscillation_count == o.oscillation_count
+    np.testing.assert_allclose(r.hessian, o.hessian, rtol=0, atol=1e-6 * max(1.0, np.abs(o.hessian).max()))
+
+
+@pytest.mark.parametrize("mode,ls", [(0, False), (1, False), (2, False), (3, False), (0, True)])
+def test_align_parity_small(small, oracle_grid, oracle_mod, mode, ls):
+    _print_case(f"align small mode {mode} ls {ls}")
+    gpu = _matcher(small.map_points, search_mode=mode, use_line_search=ls)
+    o = oracle_grid.align(small.scan, small.guess, oracle_mod.params(search_mode=mode, use_line_search=ls))
+    r = gpu.align(small.scan, small.guess)
+    _compare_align(r, o, small.scan.shape[0])
+
+
+def test_align_workgroup_counts_agree(small, oracle_grid, oracle_mod):
+    """One workgroup per alignment (block barrier) and many (slot barrier) give the oracle's answer."""
+    o = oracle_grid.align(small.scan, small.guess, oracle_mod.params())
+    for w in (1, 3, 16):
+        _print_case(f"align small W={w}")
+        r = _matcher(small.map_points, workgroups_per_alignment=w).align(small.scan, small.guess)
+        _compare_align(r, o, small.scan.shape[0])
+
+
+def test_scoring_parity(gpu_small, small, oracle_grid, oracle_mod):
+    _print_case("scoring")
+    p = oracle_mod.params()
+    pose = small.true_pose + np.array([0.05, -0.04, 0.02, 0.001, -0.002, 0.003])
+    assert gpu_small.evaluate_nvtl(small.scan, pose) == pytest.approx(oracle_grid.nvtl(small.scan, pose, p),
+                                                                       abs=1e-10)
+    assert gpu_small.evaluate_transform_probability(small.scan, pose) == pytest.approx(
+        oracle_grid.transform_probability(small.scan, pose, p), abs=1e-10)
+    np.testing.assert_allclose(gpu_small.compute_per_point_scores(small.scan, pose),
+                               oracle_grid.per_point_scores(small.scan, pose, p), rtol=1e-12, atol=1e-12)
+    poses = [pose + np.array([d, 0, 0, 0, 0, 0]) for d in np.linspace(-1, 1, 9)]
+    np.testing.assert_allclose(gpu_small.evaluate_nvtl_batch(small.scan, poses),
+                               [oracle_grid.nvtl_cell27(small.scan, q, p) for q in poses], rtol=1e-12)
+
+
+def test_align_batch_matches_oracle(gpu_small, small, oracle_grid, oracle_mod):
+    _print_case("align batch")
+    rng = np.random.default_rng(21)
+    poses = [small.true_pose + np.concatenate([rng.normal(0, 0.5, 3), [0, 0, rng.normal(0, 0.05)]])
+             for _ in range(12)]
+    res = gpu_small.align_batch(small.scan, poses)
+    p = oracle_mod.params()
+    for r, q in zip(res, poses):
+        _compare_align(r, oracle_grid.align(small.scan, q, p), small.scan.shape[0])
+
+
+def test_align_scans_uneven(gpu_small, small, oracle_grid, oracle_mod):
+    _print_case("align scans")
+    rng = np.random.default_rng(22)
+    scans = []
+    for k, n in enumerate((50, 400, 1300, small.scan.shape[0])):
+        sel = np.sort(rng.choice(small.scan.shape[0], n, replace=False))
+        scans.append((small.scan[sel], small.guess + np.array([0.1 * k, 0, 0, 0, 0, 0])))
+    res = gpu_small.align_parallel_scans(scans)
+    p = oracle_mod.params()
+    for r, (s, q) in zip(res, scans):
+        _compare_align(r, oracle_grid.align(s, q, p), s.shape[0])
+
+
+def test_error_classes(small):
+    _print_case("errors")
+    from cuda_ndt_matcher_amd import NdtError
+
+    m = _matcher()
+    with pytest.raises(NdtError) as e:
+        m.align(small.scan, small.guess)
+    assert e.value.code == -3
+    with pytest.raises(NdtError) as e:
+        m.set_target(np.zeros((0, 3), np.float32))
+    assert e.value.code == -2
+    with pytest.raises(NdtError) as e:
+        m.set_target(np.array([[0, 0, 0], [5, 5, 5]], np.float32))
+    assert e.value.code == -4
+    m.set_target(small.map_points)
+    with pytest.raises(NdtError) as e:
+        m.align(np.zeros((0, 3), np.float32), small.guess)
+    assert e.value.code == -2
+
+
+def _voxelize(pts, leaf):
+    k = np.floor(pts * (np.float32(1.0) / np.float32(leaf))).astype(np.int64)
+    keys, inv = np.unique(k, axis=0, return_inverse=True)
+    s = np.zeros((len(keys), 3))
+    c = np.zeros(len(keys))
+    np.add.at(s, inv.ravel(), pts.astype(np.float64))
+    np.add.at(c, inv.ravel(), 1)
+    return (s / c[:, None]).astype(np.float32)
+
+
+def test_half_cubic_reference_sequence(oracle_mod):
+    """ndt.rs:1490-1600 through the GPU API, against the oracle."""
+    _print_case("half cubic")
+    from cuda_ndt_matcher_amd import Isometry3
+
+    mp = npref.half_cubic() + np.array([100.0, 100.0, 0.0], np.float32)
+    scan = _voxelize(npref.half_cubic(), 1.0)
+    m = _matcher(mp)
+    og = oracle_mod.Grid(mp, 2.0)
+    good = m.align(scan, Isometry3.translation(100.0, 100.0, 0.0))
+    assert abs(good.pose.t[0] - 100) < 2 and abs(good.pose.t[1] - 100) < 2 and abs(good.pose.t[2]) < 2
+    _compare_align(good, og.align(scan, [100, 100, 0, 0, 0, 0], oracle_mod.params()), scan.shape[0])
+    bad = m.align(scan, Isometry3.translation(-100.0, -100.0, 0.0))
+    assert bad.status == "NoCorrespondences" and bad.iterations == 0 and bad.nvtl == 0.0
+    assert good.score > bad.score
+
+
+@pytest.mark.slow
+def test_full_size_parity(oracle_mod):
+    """5M-point map (about 400k voxels) and a 30k-point scan."""
+    from cuda_ndt_matcher_amd import synth
+
+    _print_case("full build")
+    w = synth.make_workload("full", 0)
+    og = oracle_mod.Grid(w.map_points, 2.0)
+    m = _matcher(w.map_points)
+    _check_build(m, w.map_points, og, oracle_mod)
+    _print_case("full align")
+    o = og.align(w.scan, w.guess, oracle_mod.params())
+    r = m.align(w.scan, w.guess)
+    _compare_align(r, o, w.scan.shape[0])
