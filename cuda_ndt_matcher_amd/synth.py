@@ -6,8 +6,8 @@ Wraps libndt_synth.so (csrc/synth.cpp).  The workloads:
            downsampled at 1.0 m (about 1.5k points); guess N(0, 0.5 m) on
            x, y, z and N(0, 2 deg) on yaw.
 * full   : 1 x 1 km, boxes and poles scaled by area (1000 / 2500), 5M map
-           points; 128-beam lidar, 1800 azimuth steps, downsampled to about
-           30k points; guess N(0, 1.0 m) on x, y, z and N(0, 3 deg) on roll,
+           points; 128-beam lidar, 1800 azimuth steps, downsampled at 0.3 m to
+           about 29k points; guess N(0, 1.0 m) on x, y, z and N(0, 3 deg) on roll,
            pitch and yaw.
 """
 from __future__ import annotations
@@ -105,7 +105,7 @@ class Workload:
 WORKLOADS = {
     # name: (extent, boxes, poles, map points, beams, leaf, sigma_t, sigma_deg, rpy perturbed)
     "small": (200.0, 40, 100, 200_000, 32, 1.0, 0.5, 2.0, False),
-    "full": (1000.0, 1000, 2500, 5_000_000, 128, 0.7, 1.0, 3.0, True),
+    "full": (1000.0, 1000, 2500, 5_000_000, 128, 0.3, 1.0, 3.0, True),
 }
 
 

@@ -809,7 +809,7 @@ double nvtl_impl(const oracle_grid* g, const float* xyz, size_t n, const double*
     double x[3];
     iso_transform(q, pose, pt, x);
     const float xf[3] = {(float)x[0], (float)x[1], (float)x[2]};
-    search(g, p->search_mode, xf, nb);
+    radius_search(g, xf, nb);  // scoring always uses the radius search (nvtl.rs:241, metrics.rs:94)
     if (nb.empty()) continue;
     double mx = -std::numeric_limits<double>::infinity();
     for (int32_t vi : nb) mx = std::max(mx, point_score(x, g->voxels[vi], gs, p->fp64_voxels != 0));
@@ -977,6 +977,18 @@ void oracle_align(const oracle_grid* g, const float* xyz, size_t n, const double
     for (int i = 0; i < 6; ++i) grad[i] = d.g[i] + rg[i];
     for (int i = 0; i < 6; ++i)
       for (int j = 0; j < 6; ++j) H[i * 6 + j] = d.H[i][j] + rH[i * 6 + j];
+    if (p->noise > 0.0) {
+      auto u = [&](uint64_t k) {
+        uint64_t x = ((uint64_t)p->noise_seed << 40) ^ ((uint64_t)it << 20) ^ k;
+        x += 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        x ^= x >> 31;
+        return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+      };
+      for (int i = 0; i < 6; ++i) grad[i] *= 1.0 + p->noise * u(i);
+      for (int i = 0; i < 36; ++i) H[i] *= 1.0 + p->noise * u(64 + i);
+    }
     std::memcpy(last_H, H, sizeof(H));
     if (score > best_score) {  // solver.rs:189-193
       best_score = score;
@@ -1054,7 +1066,7 @@ double oracle_transform_probability(const oracle_grid* g, const float* xyz, size
     double x[3];
     iso_transform(q, pose, pt, x);
     const float xf[3] = {(float)x[0], (float)x[1], (float)x[2]};
-    search(g, p->search_mode, xf, nb);
+    radius_search(g, xf, nb);  // scoring always uses the radius search (nvtl.rs:241, metrics.rs:94)
     for (int32_t vi : nb) {
       total += point_score(x, g->voxels[vi], gs, p->fp64_voxels != 0);
       ++corr;
@@ -1075,7 +1087,7 @@ void oracle_per_point_scores(const oracle_grid* g, const float* xyz, size_t n, c
     double x[3];
     iso_transform(q, pose, pt, x);
     const float xf[3] = {(float)x[0], (float)x[1], (float)x[2]};
-    search(g, p->search_mode, xf, nb);
+    radius_search(g, xf, nb);  // scoring always uses the radius search (nvtl.rs:241, metrics.rs:94)
     double s = 0.0;
     for (int32_t vi : nb) s += point_score(x, g->voxels[vi], gs, p->fp64_voxels != 0);
     out[i] = s;

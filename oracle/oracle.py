@@ -34,6 +34,8 @@ class Params(C.Structure):
         ("has_regularization_pose", C.c_int32),
         ("regularization_translation", C.c_double * 3),
         ("fp64_voxels", C.c_int32),
+        ("noise", C.c_double),
+        ("noise_seed", C.c_uint32),
     ]
 
 
@@ -121,7 +123,8 @@ def _pose(pose) -> np.ndarray:
 
 def params(resolution=2.0, outlier_ratio=0.55, max_iterations=30, trans_epsilon=0.01, step_size=0.1,
            use_line_search=False, search_mode=SEARCH_KDTREE, distance_metric=0, regularization_enabled=False,
-           regularization_scale_factor=0.01, regularization_translation=None, fp64_voxels=False) -> Params:
+           regularization_scale_factor=0.01, regularization_translation=None, fp64_voxels=False, noise=0.0,
+           noise_seed=0) -> Params:
     p = Params()
     p.resolution = resolution
     p.outlier_ratio = outlier_ratio
@@ -138,6 +141,8 @@ def params(resolution=2.0, outlier_ratio=0.55, max_iterations=30, trans_epsilon=
         for i in range(3):
             p.regularization_translation[i] = float(regularization_translation[i])
     p.fp64_voxels = int(fp64_voxels)
+    p.noise = noise
+    p.noise_seed = noise_seed
     return p
 
 
