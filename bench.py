@@ -65,6 +65,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--phases", action="store_true", help="also report the Newton kernel's phase profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -78,7 +79,8 @@ def main():
     wl = synth.make_workload("full", args.seed)
     n = wl.scan.shape[0]
     print(f"[bench] map {wl.map_points.shape[0]} pts, scan {n} pts", file=sys.stderr, flush=True)
-    m = NdtScanMatcher.builder().resolution(2.0).max_iterations(30).transformation_epsilon(0.01).build()
+    m = (NdtScanMatcher.builder().resolution(2.0).max_iterations(30).transformation_epsilon(0.01)
+         .profile_phases(args.phases).build())
     m.set_target(wl.map_points)
     t = m.timing()
     build_ms = t["build_ms"]
@@ -109,6 +111,9 @@ def main():
           f"corr {r.num_correspondences}, status {r.status}", file=sys.stderr, flush=True)
 
     extra = {}
+    if args.phases:
+        extra["phases_us"] = m.phase_profile()
+        print(f"[bench] phases {extra['phases_us']}", file=sys.stderr, flush=True)
     if args.particles:
         rng = np.random.default_rng(args.seed + 7)
         parts = [wl.true_pose + np.array([rng.uniform(-2, 2), rng.uniform(-2, 2), 0.0, rng.normal(0, math.radians(1)),

@@ -41,6 +41,7 @@ class NdtConfig(C.Structure):
         ("search_mode", C.c_int32),
         ("precision", C.c_int32),
         ("workgroups_per_alignment", C.c_int32),
+        ("profile_phases", C.c_int32),
     ]
 
 
@@ -71,6 +72,22 @@ class NdtTiming(C.Structure):
     ]
 
 
+class NdtPhaseProfile(C.Structure):
+    _fields_ = [
+        ("derivative_us", C.c_float),
+        ("exchange_us", C.c_float),
+        ("solve_us", C.c_float),
+        ("line_search_us", C.c_float),
+        ("nvtl_us", C.c_float),
+        ("total_us", C.c_float),
+        ("iterations", C.c_uint32),
+        ("line_search_evaluations", C.c_uint32),
+        ("slowest_derivative_us", C.c_float),
+        ("solve_setup_us", C.c_float),
+        ("solve_step_us", C.c_float),
+    ]
+
+
 # Every symbol declared in include/ndt_hip.h.
 EXPORTS = [
     "ndt_hip_config_default", "ndt_hip_create", "ndt_hip_destroy", "ndt_hip_last_error",
@@ -80,7 +97,7 @@ EXPORTS = [
     "ndt_hip_align_scans", "ndt_hip_evaluate_nvtl", "ndt_hip_evaluate_transform_probability",
     "ndt_hip_per_point_scores", "ndt_hip_evaluate_nvtl_batch", "ndt_hip_set_regularization_pose",
     "ndt_hip_clear_regularization_pose", "ndt_hip_laplace_covariance_xy", "ndt_hip_laplace_covariance_6x6",
-    "ndt_hip_get_timing",
+    "ndt_hip_get_timing", "ndt_hip_get_phase_profile",
 ]
 
 _lib = None
@@ -132,6 +149,7 @@ def load():
     lib.ndt_hip_laplace_covariance_xy.argtypes = [f64p, f64p]
     lib.ndt_hip_laplace_covariance_6x6.argtypes = [f64p, f64p]
     lib.ndt_hip_get_timing.argtypes = [vp, C.POINTER(NdtTiming)]
+    lib.ndt_hip_get_phase_profile.argtypes = [vp, C.POINTER(NdtPhaseProfile)]
     _lib = lib
     return lib
 

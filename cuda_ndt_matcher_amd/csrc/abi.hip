@@ -24,7 +24,9 @@ struct ndt_handle {
   hipStream_t stream = nullptr;
   ndt::DeviceGrid grid;
   std::string err;
-  DevBuf scan, jobs, results, ws, poses, partials, out, lookup_q, lookup_c, lookup_i;
+  DevBuf scan, jobs, results, ws, poses, partials, out, lookup_q, lookup_c, lookup_i, phase;
+  ndt_phase_profile phase_profile{};
+  int wall_clock_khz = 0;
   std::vector<const float*> scan_ptrs;
   ndt_result* host_results = nullptr;
   size_t host_results_cap = 0;
@@ -79,6 +81,7 @@ ndt::NewtonConfig newton_config(const ndt_handle* h, int wg) {
   c.reg_active = (h->cfg.regularization_enabled && h->has_reg_pose) ? 1 : 0;
   c.precision = h->cfg.precision;
   c.wg_per_slot = wg;
+  c.phase = h->cfg.profile_phases ? reinterpret_cast<unsigned long long*>(h->phase.ptr) : nullptr;
   return c;
 }
 
@@ -106,13 +109,13 @@ int ensure_host_results(ndt_handle* h, size_t m) {
   return NDT_OK;
 }
 
-// Workgroups per alignment: about two points per thread, capped so that all
+// Workgroups per alignment: about one point per thread, capped so that all
 // slots' workgroups are co-resident (the slot barrier needs that).
 int choose_wg(const ndt_handle* h, size_t n_max, size_t m) {
   const int cap = std::max(1, h->coop_capacity);
   int per_slot_cap = (int)std::max<size_t>(1, (size_t)cap / std::max<size_t>(1, m));
   if (h->cfg.workgroups_per_alignment > 0) return std::min(h->cfg.workgroups_per_alignment, per_slot_cap);
-  const size_t want = (n_max + 511) / 512;
+  const size_t want = (n_max + 255) / 256;
   return (int)std::max<size_t>(1, std::min<size_t>(want, (size_t)per_slot_cap));
 }
 
@@ -138,6 +141,10 @@ int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, co
   NDT_HIP(h->jobs.reserve(m * sizeof(ndt::SlotJob)), "alloc jobs");
   NDT_HIP(h->results.reserve(m * sizeof(ndt_result)), "alloc results");
   NDT_HIP(h->ws.reserve(ndt::newton_workspace_bytes((int)m, W)), "alloc workspace");
+  if (h->cfg.profile_phases) {
+    NDT_HIP(h->phase.reserve(16 * sizeof(unsigned long long)), "alloc phase");
+    NDT_HIP(hipMemsetAsync(h->phase.ptr, 0, 16 * sizeof(unsigned long long), h->stream), "clear phase");
+  }
   int rc = ensure_host_results(h, m);
   if (rc) return rc;
   NDT_HIP(hipMemcpyAsync(h->jobs.ptr, jobs.data(), m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
@@ -157,6 +164,23 @@ int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, co
   h->timing.align_workgroups = (uint32_t)(W * m);
   h->timing.align_slots = (uint32_t)m;
   std::memcpy(out, h->host_results, m * sizeof(ndt_result));
+  if (h->cfg.profile_phases) {
+    unsigned long long ph[16];
+    NDT_HIP(hipMemcpy(ph, h->phase.ptr, sizeof(ph), hipMemcpyDeviceToHost), "copy phases");
+    const double us = h->wall_clock_khz > 0 ? 1e3 / (double)h->wall_clock_khz : 0.0;
+    ndt_phase_profile& p = h->phase_profile;
+    p.derivative_us = (float)(ph[0] * us);
+    p.exchange_us = (float)(ph[1] * us);
+    p.solve_us = (float)(ph[2] * us);
+    p.line_search_us = (float)(ph[3] * us);
+    p.nvtl_us = (float)(ph[4] * us);
+    p.total_us = (float)(ph[5] * us);
+    p.iterations = (uint32_t)ph[6];
+    p.line_search_evaluations = (uint32_t)ph[7];
+    p.slowest_derivative_us = (float)(ph[8] * us);
+    p.solve_setup_us = (float)(ph[9] * us);
+    p.solve_step_us = (float)(ph[10] * us);
+  }
   return NDT_OK;
 }
 
@@ -234,6 +258,7 @@ void ndt_hip_config_default(ndt_config* c) {
   c->search_mode = NDT_SEARCH_KDTREE;
   c->precision = NDT_PRECISION_FP64;
   c->workgroups_per_alignment = 0;
+  c->profile_phases = 0;
 }
 
 int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
@@ -260,6 +285,7 @@ int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
     return NDT_ERR_HIP;
   }
   h->coop_capacity = ndt::newton_max_coresident_workgroups();
+  (void)hipDeviceGetAttribute(&h->wall_clock_khz, hipDeviceAttributeWallClockRate, h->device);
   *out = h;
   return NDT_OK;
 }
@@ -271,8 +297,8 @@ void ndt_hip_destroy(ndt_handle* h) {
   ndt::DeviceGrid& G = h->grid;
   for (DevBuf* b : {&G.owned_points, &G.sort_keys_a, &G.sort_keys_b, &G.sort_vals_a, &G.sort_vals_b, &G.hist,
                     &G.status, &G.tile_heads, &G.seg_start, &G.scalars, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
-                    &G.valid, &G.table, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
-                    &h->lookup_q, &h->lookup_c, &h->lookup_i})
+                    &G.valid, &G.table, &G.bitmap, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
+                    &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
   if (G.host_scalars) (void)hipHostFree(G.host_scalars);
   if (h->host_results) (void)hipHostFree(h->host_results);
@@ -600,6 +626,12 @@ int ndt_hip_laplace_covariance_6x6(const double hess[36], double cov[36]) {
   }
   for (int i = 0; i < 6; ++i)
     for (int j = 0; j < 6; ++j) cov[i * 6 + j] = -a[i][6 + j];
+  return NDT_OK;
+}
+
+int ndt_hip_get_phase_profile(const ndt_handle* h, ndt_phase_profile* out) {
+  if (!h || !out) return NDT_ERR_INVALID_ARGUMENT;
+  *out = h->phase_profile;
   return NDT_OK;
 }
 

@@ -17,12 +17,31 @@ constexpr int32_t kCoordBias = 1 << 20;  // voxel coordinates must lie in [-2^20
 // Voxel grid as the kernels see it.  Voxel slots are in Morton-key order; a
 // slot whose voxel was dropped (too few points, degenerate covariance) is
 // never inserted in the hash and so never visited.
+//
+// Lookup structures:
+//  - an open-addressing hash keyed by the packed voxel coordinate, 32-byte
+//    entries {key, slot, mean} (linear probing, load <= 0.5), so one probe
+//    returns the mean needed for the radius test;
+//  - an occupancy bitmap over the voxel box padded by two cells on every
+//    side (one bit per cell) when that box has fewer than 2^32 cells; it
+//    answers "is this cell occupied" for the 3x3x3 block of a query with nine
+//    independent word loads, so the hash is probed only for occupied cells.
+struct __align__(32) HashEntry {
+  unsigned long long key;
+  uint32_t slot;
+  float mx, my, mz;
+  uint32_t pad[2];
+};
+
 struct GridView {
-  const float4* vox;     // 3 x float4 per slot: {mx,my,mz,C00},{C01,C02,C10,C11},{C12,C20,C21,C22}
-  const float* axis;     // 3 floats per slot: principal axis (point-to-plane metric)
-  const uint4* table;    // open-addressing hash: {key lo, key hi, slot, 0}
-  uint32_t table_mask;   // capacity - 1 (capacity is a power of two, load <= 0.5)
-  int32_t cmin[3];       // bounding box of the occupied voxel coordinates
+  const float4* vox;        // 3 x float4 per slot: {mx,my,mz,C00},{C01,C02,C10,C11},{C12,C20,C21,C22}
+  const float* axis;        // 3 floats per slot: principal axis (point-to-plane metric)
+  const HashEntry* table;   // capacity is a power of two
+  uint32_t table_mask;      // capacity - 1
+  const uint32_t* bitmap;   // null when the padded box is too large
+  int32_t borg[3];          // bitmap origin (cmin - 2)
+  uint32_t bdim[3];         // bitmap extent (cmax - cmin + 5)
+  int32_t cmin[3];          // bounding box of the occupied voxel coordinates
   int32_t cmax[3];
   float res;
 };
@@ -42,19 +61,103 @@ __host__ __device__ inline uint64_t mix64(uint64_t k) {
   return k;
 }
 
+__device__ inline bool in_box(const GridView& g, int32_t x, int32_t y, int32_t z) {
+  return x >= g.cmin[0] && x <= g.cmax[0] && y >= g.cmin[1] && y <= g.cmax[1] && z >= g.cmin[2] && z <= g.cmax[2];
+}
+
+// Entry index of cell (x, y, z), or -1.
 __device__ inline int32_t hash_find(const GridView& g, int32_t x, int32_t y, int32_t z) {
-  if (x < g.cmin[0] || x > g.cmax[0] || y < g.cmin[1] || y > g.cmax[1] || z < g.cmin[2] || z > g.cmax[2])
-    return -1;
+  if (!in_box(g, x, y, z)) return -1;
   const uint64_t key = pack_coord(x, y, z);
   uint32_t h = (uint32_t)mix64(key) & g.table_mask;
 #pragma unroll 1
   for (;;) {
-    const uint4 e = g.table[h];
-    const uint64_t k = (uint64_t)e.x | ((uint64_t)e.y << 32);
-    if (k == key) return (int32_t)e.z;
+    const unsigned long long k = g.table[h].key;
+    if (k == key) return (int32_t)h;
     if (k == kEmptyKey) return -1;
     h = (h + 1) & g.table_mask;
   }
+}
+
+// Cell index within the 3x3x3 block: c = (dx+1)*9 + (dy+1)*3 + (dz+1).
+__device__ inline void cell_offset(int c, int& dx, int& dy, int& dz) {
+  dx = c / 9 - 1;
+  dy = (c / 3) % 3 - 1;
+  dz = c % 3 - 1;
+}
+
+// Occupancy of the 3x3x3 block around (kx, ky, kz) as a 27-bit mask.  The
+// centre must lie in [cmin-1, cmax+1] so the block is inside the padded box.
+__device__ inline uint32_t occupancy27(const GridView& g, int32_t kx, int32_t ky, int32_t kz) {
+  uint32_t m = 0;
+  const uint64_t x0 = (uint64_t)(kx - 1 - g.borg[0]);
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int dy = r / 3 - 1, dz = r % 3 - 1;
+    const uint64_t lin = x0 + (uint64_t)g.bdim[0] * ((uint64_t)(ky + dy - g.borg[1]) +
+                                                      (uint64_t)g.bdim[1] * (uint64_t)(kz + dz - g.borg[2]));
+    const uint64_t w = lin >> 5;
+    const uint64_t two = (uint64_t)g.bitmap[w] | ((uint64_t)g.bitmap[w + 1] << 32);
+    const uint32_t b3 = (uint32_t)(two >> (lin & 31)) & 7u;  // bit0: x-1, bit1: x, bit2: x+1
+    m |= ((b3 & 1u) << r) | (((b3 >> 1) & 1u) << (9 + r)) | (((b3 >> 2) & 1u) << (18 + r));
+  }
+  return m;
+}
+
+// Visits the voxels of the cells set in `cand` (27-bit mask around k),
+// probing two cells at a time so their table loads overlap.  `accept(e)`
+// decides from the entry (mean) whether to visit; visit(slot, mean).
+template <class A, class F>
+__device__ inline void probe_cells(const GridView& g, int32_t kx, int32_t ky, int32_t kz, uint32_t cand, A&& accept,
+                                   F&& visit) {
+#pragma unroll 1
+  while (cand) {
+    const int c0 = __ffs(cand) - 1;
+    cand &= cand - 1;
+    int c1 = -1;
+    if (cand) {
+      c1 = __ffs(cand) - 1;
+      cand &= cand - 1;
+    }
+    int dx, dy, dz;
+    cell_offset(c0, dx, dy, dz);
+    const uint64_t k0 = pack_coord(kx + dx, ky + dy, kz + dz);
+    uint32_t h0 = (uint32_t)mix64(k0) & g.table_mask;
+    uint64_t k1 = 0;
+    uint32_t h1 = 0;
+    if (c1 >= 0) {
+      cell_offset(c1, dx, dy, dz);
+      k1 = pack_coord(kx + dx, ky + dy, kz + dz);
+      h1 = (uint32_t)mix64(k1) & g.table_mask;
+    }
+    HashEntry e0 = g.table[h0];
+    HashEntry e1;
+    if (c1 >= 0) e1 = g.table[h1];
+#pragma unroll 1
+    while (e0.key != k0 && e0.key != kEmptyKey) {
+      h0 = (h0 + 1) & g.table_mask;
+      e0 = g.table[h0];
+    }
+    if (e0.key == k0 && accept(e0)) visit((int32_t)e0.slot, e0.mx, e0.my, e0.mz);
+    if (c1 >= 0) {
+#pragma unroll 1
+      while (e1.key != k1 && e1.key != kEmptyKey) {
+        h1 = (h1 + 1) & g.table_mask;
+        e1 = g.table[h1];
+      }
+      if (e1.key == k1 && accept(e1)) visit((int32_t)e1.slot, e1.mx, e1.my, e1.mz);
+    }
+  }
+}
+
+// Block mask restricted to cells that can exist: all 27 when the centre's
+// block lies in the padded box, and the bitmap's answer when there is one.
+__device__ inline bool block_mask(const GridView& g, int32_t kx, int32_t ky, int32_t kz, uint32_t& mask) {
+  if (kx < g.cmin[0] - 1 || kx > g.cmax[0] + 1 || ky < g.cmin[1] - 1 || ky > g.cmax[1] + 1 ||
+      kz < g.cmin[2] - 1 || kz > g.cmax[2] + 1)
+    return false;  // no occupied cell within one of k
+  if (g.bitmap) mask &= occupancy27(g, kx, ky, kz);
+  return mask != 0;
 }
 
 // Rust `f32 as i32` / `f64 as i32`: saturating, NaN -> 0.
@@ -88,7 +191,8 @@ __device__ inline float dist2_f32(float qx, float qy, float qz, float mx, float 
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour enumeration.  `visit(slot)` is called for each neighbouring voxel.
+// Neighbour enumeration.  visit(slot, mx, my, mz) is called for each
+// neighbouring voxel.
 //
 // Radius mode: every voxel whose f32 mean is within `res` of q (strict <).
 // A voxel's mean lies in the closed box of its own cell, so a mean within
@@ -101,53 +205,44 @@ __device__ inline void for_each_radius(const GridView& g, float qx, float qy, fl
   const int32_t kx = sat_i32d(floor((double)qx / res));
   const int32_t ky = sat_i32d(floor((double)qy / res));
   const int32_t kz = sat_i32d(floor((double)qz / res));
-  const float r2 = __fmul_rn(g.res, g.res);
   // Distance from q to the low/high faces of its own cell, per axis.
   const double lox = (double)qx - kx * res, hix = (kx + 1) * res - (double)qx;
   const double loy = (double)qy - ky * res, hiy = (ky + 1) * res - (double)qy;
   const double loz = (double)qz - kz * res, hiz = (kz + 1) * res - (double)qz;
   const double lim = res * res * (1.0 + 1e-6);
-#pragma unroll 1
+  uint32_t mask = 0;
+#pragma unroll
   for (int c = 0; c < 27; ++c) {
     const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
     const double ex = dx < 0 ? lox : (dx > 0 ? hix : 0.0);
     const double ey = dy < 0 ? loy : (dy > 0 ? hiy : 0.0);
     const double ez = dz < 0 ? loz : (dz > 0 ? hiz : 0.0);
-    if (ex * ex + ey * ey + ez * ez > lim) continue;
-    const int32_t s = hash_find(g, kx + dx, ky + dy, kz + dz);
-    if (s < 0) continue;
-    const float4 m = g.vox[3 * s];
-    if (dist2_f32(qx, qy, qz, m.x, m.y, m.z) < r2) visit(s, m);
+    if (ex * ex + ey * ey + ez * ez <= lim) mask |= 1u << c;
   }
+  if (!block_mask(g, kx, ky, kz, mask)) return;
+  const float r2 = __fmul_rn(g.res, g.res);
+  probe_cells(
+      g, kx, ky, kz, mask, [&](const HashEntry& e) { return dist2_f32(qx, qy, qz, e.mx, e.my, e.mz) < r2; },
+      visit);
 }
 
 // Autoware pclomp direct searches: cell = floor(q/res) in f32; DIRECT1 the
 // cell, DIRECT7 the cell and its 6 faces, DIRECT26 the 26 cells around it
 // (PCL getAllNeighborCellIndices, which leaves the centre out).
+__host__ __device__ constexpr uint32_t direct_mask(int mode) {
+  // bit c = (dx+1)*9 + (dy+1)*3 + (dz+1); centre 13, faces 4, 22, 10, 16, 12, 14
+  return mode == 3 ? (1u << 13)
+                   : (mode == 2 ? ((1u << 13) | (1u << 22) | (1u << 4) | (1u << 16) | (1u << 10) | (1u << 14) |
+                                   (1u << 12))
+                                : (((1u << 27) - 1) & ~(1u << 13)));
+}
+
 template <class F>
 __device__ inline void for_each_direct(const GridView& g, int mode, float qx, float qy, float qz, F&& visit) {
   const int32_t kx = voxel_key(qx, g.res), ky = voxel_key(qy, g.res), kz = voxel_key(qz, g.res);
-  // DIRECT7 offsets in pclomp's order: centre, +x, -x, +y, -y, +z, -z.
-  const int count = mode == 3 ? 1 : (mode == 2 ? 7 : 26);
-#pragma unroll 1
-  for (int c = 0; c < count; ++c) {
-    int dx = 0, dy = 0, dz = 0;
-    if (mode == 2) {
-      const int axis = (c + 1) / 2, sign = (c & 1) ? 1 : -1;
-      if (c > 0) {
-        if (axis == 1) dx = sign;
-        else if (axis == 2) dy = sign;
-        else dz = sign;
-      }
-    } else if (mode == 1) {
-      const int k = c < 13 ? c : c + 1;  // skip the centre (index 13 of 27)
-      dx = k / 9 - 1;
-      dy = (k / 3) % 3 - 1;
-      dz = k % 3 - 1;
-    }
-    const int32_t s = hash_find(g, kx + dx, ky + dy, kz + dz);
-    if (s >= 0) visit(s, g.vox[3 * s]);
-  }
+  uint32_t mask = direct_mask(mode);
+  if (!block_mask(g, kx, ky, kz, mask)) return;
+  probe_cells(g, kx, ky, kz, mask, [](const HashEntry&) { return true; }, visit);
 }
 
 template <class F>
@@ -164,11 +259,9 @@ template <class F>
 __device__ inline void for_each_cell27(const GridView& g, double x, double y, double z, F&& visit) {
   const double inv = 1.0 / (double)g.res;
   const int32_t vx = sat_i32d(floor(x * inv)), vy = sat_i32d(floor(y * inv)), vz = sat_i32d(floor(z * inv));
-#pragma unroll 1
-  for (int c = 0; c < 27; ++c) {
-    const int32_t s = hash_find(g, vx + c / 9 - 1, vy + (c / 3) % 3 - 1, vz + c % 3 - 1);
-    if (s >= 0) visit(s, g.vox[3 * s]);
-  }
+  uint32_t mask = (1u << 27) - 1;
+  if (!block_mask(g, vx, vy, vz, mask)) return;
+  probe_cells(g, vx, vy, vz, mask, [](const HashEntry&) { return true; }, visit);
 }
 
 // ---------------------------------------------------------------------------
@@ -264,84 +357,6 @@ constexpr int kAccSlots = 30; // + correspondences + points-with-neighbours/spar
 
 __device__ inline int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
 
-// One point-voxel pair, point-to-distribution metric: compute_derivative_single
-// (derivatives/cpu.rs:139-229).  x = transformed point - mean, C = inverse
-// covariance (row-major, as stored), J = the 3x6 point Jacobian (translation
-// columns are the identity), ht = the 15 point-Hessian terms.
-__device__ inline void pair_p2d(const double x[3], const double C[9], const double Jr[3][3], const double ht[15],
-                                double d1, double d2, bool hess, double* acc) {
-  double xc[3];
-  for (int j = 0; j < 3; ++j) xc[j] = x[0] * C[j] + x[1] * C[3 + j] + x[2] * C[6 + j];
-  const double xcx = xc[0] * x[0] + xc[1] * x[1] + xc[2] * x[2];
-  const double e_raw = exp(-d2 * xcx * 0.5);
-  double e = d2 * e_raw;
-  if (!(e >= 0.0 && e <= 1.0) || !isfinite(e)) return;  // counted but contributes zero
-  acc[0] += -d1 * e_raw;
-  e *= d1;
-  // C J: translation columns are C's columns; rotation columns C * Jr[:,c].
-  double cJ[3][6];
-  for (int r = 0; r < 3; ++r) {
-    cJ[r][0] = C[3 * r + 0];
-    cJ[r][1] = C[3 * r + 1];
-    cJ[r][2] = C[3 * r + 2];
-    for (int c = 0; c < 3; ++c) cJ[r][3 + c] = C[3 * r] * Jr[0][c] + C[3 * r + 1] * Jr[1][c] + C[3 * r + 2] * Jr[2][c];
-  }
-  double xcJ[6];
-  for (int c = 0; c < 6; ++c) xcJ[c] = x[0] * cJ[0][c] + x[1] * cJ[1][c] + x[2] * cJ[2][c];
-  for (int c = 0; c < 6; ++c) acc[1 + c] += e * xcJ[c];
-  if (!hess) return;
-  // (J^T C J)[j][i] = J[:,j] . (C J)[:,i]
-  auto JcJ = [&](int jj, int ii) -> double {
-    if (jj < 3) return cJ[jj][ii];
-    return Jr[0][jj - 3] * cJ[0][ii] + Jr[1][jj - 3] * cJ[1][ii] + Jr[2][jj - 3] * cJ[2][ii];
-  };
-  // x^T C * d2T/dp_i dp_j for the rotation block (a..f with their zero rows).
-  const double xa = xc[1] * ht[0] + xc[2] * ht[1];
-  const double xb = xc[1] * ht[2] + xc[2] * ht[3];
-  const double xcc = xc[1] * ht[4] + xc[2] * ht[5];
-  const double xd = xc[0] * ht[6] + xc[1] * ht[7] + xc[2] * ht[8];
-  const double xe = xc[0] * ht[9] + xc[1] * ht[10] + xc[2] * ht[11];
-  const double xf = xc[0] * ht[12] + xc[1] * ht[13] + xc[2] * ht[14];
-  const double xh[3][3] = {{xa, xb, xcc}, {xb, xd, xe}, {xcc, xe, xf}};
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j <= i; ++j) {
-      const double ph = (i >= 3 && j >= 3) ? xh[i - 3][j - 3] : 0.0;
-      acc[7 + tri(i, j)] += e * (-d2 * xcJ[i] * xcJ[j] + ph + JcJ(j, i));
-    }
-}
-
-// Point-to-plane metric: compute_derivative_point_to_plane (derivatives/cpu.rs:246-329).
-__device__ inline void pair_p2plane(const double x[3], const double n[3], const double Jr[3][3], const double ht[15],
-                                    double d1, double d2, bool hess, double* acc) {
-  const double d = x[0] * n[0] + x[1] * n[1] + x[2] * n[2];
-  const double dsq = d * d;
-  const double e_raw = exp(-d2 * dsq * 0.5);
-  const double e = d1 * d2 * e_raw;
-  const double normalized = d2 * e_raw;
-  if (!(normalized >= 0.0 && normalized <= 1.0) || !isfinite(normalized)) return;
-  acc[0] += -d1 * e_raw;
-  double dd[6];
-  dd[0] = n[0];
-  dd[1] = n[1];
-  dd[2] = n[2];
-  for (int c = 0; c < 3; ++c) dd[3 + c] = n[0] * Jr[0][c] + n[1] * Jr[1][c] + n[2] * Jr[2][c];
-  for (int c = 0; c < 6; ++c) acc[1 + c] += e * d * dd[c];
-  if (!hess) return;
-  const double factor = -d2 * dsq + 1.0;
-  const double na = n[1] * ht[0] + n[2] * ht[1];
-  const double nb = n[1] * ht[2] + n[2] * ht[3];
-  const double nc = n[1] * ht[4] + n[2] * ht[5];
-  const double nd = n[0] * ht[6] + n[1] * ht[7] + n[2] * ht[8];
-  const double ne = n[0] * ht[9] + n[1] * ht[10] + n[2] * ht[11];
-  const double nf = n[0] * ht[12] + n[1] * ht[13] + n[2] * ht[14];
-  const double nh[3][3] = {{na, nb, nc}, {nb, nd, ne}, {nc, ne, nf}};
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j <= i; ++j) {
-      const double ph = (i >= 3 && j >= 3) ? nh[i - 3][j - 3] : 0.0;
-      acc[7 + tri(i, j)] += e * factor * (dd[i] * dd[j]) + e * d * ph;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Wavefront / workgroup reductions (f64).
 __device__ inline double wave_sum(double v) {
@@ -415,73 +430,185 @@ __device__ inline void sym_eigen(const double* a_in, double* ev, double* v) {
   for (int i = 0; i < N; ++i) ev[i] = a[i][i];
 }
 
-// Cholesky of a (6x6, lower triangle read); returns false if not positive definite.
-__device__ inline bool cholesky6(const double* a, double L[6][6]) {
+// ---------------------------------------------------------------------------
+// Workgroup-cooperative 6x6 Newton step in LDS.
+//
+// newton_step_negative_definite (optimization/newton.rs:160-176): when every
+// eigenvalue of H is negative the reference solves H d = -g with an SVD
+// pseudo-inverse (tolerance 1e-10); otherwise it clamps eigenvalues above
+// -1e-3 to -1e-3 and solves with the rebuilt matrix.  Here:
+//  - thread 0 factors -H - 1e-10 I (Cholesky).  Success means every
+//    eigenvalue of H is below -1e-10, so the pseudo-inverse is the plain
+//    inverse and a Cholesky solve of (-H) d = g gives the same step;
+//  - otherwise the workgroup runs a parallel cyclic Jacobi on H: each round
+//    applies 3 disjoint rotations (a 1-factorisation of K6, 5 rounds per
+//    sweep) with one thread per matrix entry, then thread 0 applies the
+//    reference's clamp and pseudo-inverse in the eigenbasis.
+struct SolveScratch {
+  double A[36];
+  double V[36];   // eigenvectors (columns); kept as the next call's starting basis
+  double T[36];
+  double L[36];
+  double c[6];
+  double o[6];
+  int partner[6];
+  int have_basis; // V holds the eigenvectors of the previous call's matrix
+  int fast;
+  int stop;
+  int ok;
+};
+
+// Cholesky of -H - shift*I (lower triangle of H read) into L; false if not PD.
+__device__ inline bool neg_cholesky6(const double* H, double shift, double* L) {
   for (int j = 0; j < 6; ++j) {
-    double d = a[j * 6 + j];
-    for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+    double d = -H[j * 6 + j] - shift;
+    for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k];
     if (!(d > 0.0)) return false;
     const double ljj = sqrt(d);
-    L[j][j] = ljj;
+    L[j * 6 + j] = ljj;
     for (int i = j + 1; i < 6; ++i) {
-      double s = a[i * 6 + j];
-      for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
-      L[i][j] = s / ljj;
+      double s = -H[i * 6 + j];
+      for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k];
+      L[i * 6 + j] = s / ljj;
     }
   }
   return true;
 }
 
-// newton_step_negative_definite (optimization/newton.rs:160-176): Newton step
-// for a maximisation with min eigenvalue -1e-3 and SVD tolerance 1e-10.  When
-// -H - 1e-10 I is positive definite, every eigenvalue of H is below -1e-10,
-// so the reference's SVD pseudo-inverse equals the plain inverse and a
-// Cholesky solve of (-H) d = g gives the same step; otherwise the Jacobi
-// eigendecomposition applies the reference's clamp and pseudo-inverse.
-__device__ inline bool newton_step_device(const double g[6], const double H[36], double d[6]) {
-  double L[6][6];
-  double A[36];
-  for (int i = 0; i < 36; ++i) A[i] = -H[i];
-  for (int i = 0; i < 6; ++i) A[i * 6 + i] -= 1e-10;
-  if (cholesky6(A, L)) {
-    for (int i = 0; i < 36; ++i) A[i] = -H[i];
-    if (cholesky6(A, L)) {
-      double y[6];
-      for (int i = 0; i < 6; ++i) {
-        double s = g[i];
-        for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
-        y[i] = s / L[i][i];
+// Round-robin pairing of 6 indices: 5 rounds of 3 disjoint pairs.
+__constant__ static const int kJacobiPairs[5][3][2] = {
+    {{0, 1}, {2, 3}, {4, 5}}, {{0, 2}, {1, 4}, {3, 5}}, {{0, 3}, {1, 5}, {2, 4}},
+    {{0, 4}, {1, 3}, {2, 5}}, {{0, 5}, {1, 2}, {3, 4}}};
+
+// Orders LDS accesses of the lanes of one wave (no cross-wave barrier).
+__device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
+
+// All threads of the workgroup call this; wave 0 does the work.  H (36, lower
+// triangle read) and g (6) are in LDS; on return W.ok says whether a step
+// exists and d (LDS) holds it.  When W.have_basis is set the Jacobi sweeps
+// start from V^T H V with the previous call's eigenvectors V, which for the
+// slowly changing Hessians of successive Newton iterations is nearly diagonal.
+__device__ inline void newton_step_wg(const double* H, const double* g, double* d, SolveScratch& W) {
+  const int lane = threadIdx.x;
+  if (lane < kWave) {
+    if (lane == 0) {
+      W.fast = 0;
+      W.ok = 0;
+      if (neg_cholesky6(H, 1e-10, W.L) && neg_cholesky6(H, 0.0, W.L)) {
+        double y[6];
+        for (int i = 0; i < 6; ++i) {
+          double s = g[i];
+          for (int k = 0; k < i; ++k) s -= W.L[i * 6 + k] * y[k];
+          y[i] = s / W.L[i * 6 + i];
+        }
+        for (int i = 5; i >= 0; --i) {
+          double s = y[i];
+          for (int k = i + 1; k < 6; ++k) s -= W.L[k * 6 + i] * d[k];
+          d[i] = s / W.L[i * 6 + i];
+        }
+        W.fast = 1;
+        W.ok = 1;
       }
-      for (int i = 5; i >= 0; --i) {
-        double s = y[i];
-        for (int k = i + 1; k < 6; ++k) s -= L[k][i] * d[k];
-        d[i] = s / L[i][i];
+    }
+    wave_lds_fence();
+    if (!W.fast) {
+      const int i = lane / 6, j = lane % 6;
+      if (W.have_basis) {
+        // T = Hs V, A = V^T T (Hs: H symmetrised from its lower triangle)
+        if (lane < 36) {
+          double t = 0.0;
+          for (int k = 0; k < 6; ++k) t += (i >= k ? H[i * 6 + k] : H[k * 6 + i]) * W.V[k * 6 + j];
+          W.T[lane] = t;
+        }
+        wave_lds_fence();
+        double a = 0.0;
+        if (lane < 36)
+          for (int k = 0; k < 6; ++k) a += W.V[k * 6 + i] * W.T[k * 6 + j];
+        wave_lds_fence();
+        if (lane < 36) W.T[lane] = a;
+        wave_lds_fence();
+        if (lane < 36) W.A[lane] = i >= j ? W.T[i * 6 + j] : W.T[j * 6 + i];
+      } else if (lane < 36) {
+        W.A[lane] = i >= j ? H[i * 6 + j] : H[j * 6 + i];
+        W.V[lane] = i == j ? 1.0 : 0.0;
       }
-      return true;
+      wave_lds_fence();
+      for (int sweep = 0; sweep < 60; ++sweep) {
+        if (lane == 0) {
+          double off = 0.0, diag = 0.0;
+          for (int a = 0; a < 6; ++a) {
+            diag += W.A[a * 7] * W.A[a * 7];
+            for (int b = a + 1; b < 6; ++b) off += W.A[a * 6 + b] * W.A[a * 6 + b];
+          }
+          W.stop = (off == 0.0 || off <= 1e-34 * diag) ? 1 : 0;
+        }
+        wave_lds_fence();
+        if (W.stop) break;
+        for (int r = 0; r < 5; ++r) {
+          if (lane < 3) {
+            const int p = kJacobiPairs[r][lane][0], q = kJacobiPairs[r][lane][1];
+            const double apq = W.A[p * 6 + q];
+            double c = 1.0, s = 0.0;
+            if (apq != 0.0) {
+              // t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)) with theta = dd / e,
+              // multiplied through by |e|: one sqrt, one division, one rsqrt.
+              const double dd = W.A[q * 7] - W.A[p * 7], e = 2.0 * apq;
+              const double t = (dd >= 0.0 ? e : -e) / (fabs(dd) + sqrt(dd * dd + e * e));
+              c = rsqrt(t * t + 1.0);
+              s = t * c;
+            }
+            W.c[p] = c;
+            W.c[q] = c;
+            W.o[p] = -s;  // G[q][p]
+            W.o[q] = s;   // G[p][q]
+            W.partner[p] = q;
+            W.partner[q] = p;
+          }
+          wave_lds_fence();
+          if (lane < 36) {
+            const int mi = W.partner[i], mj = W.partner[j];
+            const double ci = W.c[i], oi = W.o[i], cj = W.c[j], oj = W.o[j];
+            const double an = (mi == j) ? 0.0
+                                        : ci * cj * W.A[i * 6 + j] + ci * oj * W.A[i * 6 + mj] +
+                                              oi * cj * W.A[mi * 6 + j] + oi * oj * W.A[mi * 6 + mj];
+            const double vn = W.V[i * 6 + j] * cj + W.V[i * 6 + mj] * oj;
+            wave_lds_fence();
+            W.A[lane] = an;
+            W.V[lane] = vn;
+          } else {
+            wave_lds_fence();
+          }
+          wave_lds_fence();
+        }
+      }
+      if (lane == 0) {
+        double ev[6];
+        bool neg_def = true;
+        for (int a = 0; a < 6; ++a) {
+          ev[a] = W.A[a * 7];
+          neg_def &= ev[a] < 0.0;
+        }
+        double lam[6], max_sv = 0.0;
+        for (int a = 0; a < 6; ++a) {
+          lam[a] = (!neg_def && ev[a] > -1e-3) ? -1e-3 : ev[a];
+          max_sv = fmax(max_sv, fabs(lam[a]));
+        }
+        if (max_sv >= 1e-10) {
+          for (int a = 0; a < 6; ++a) d[a] = 0.0;
+          for (int k = 0; k < 6; ++k) {
+            if (!(fabs(lam[k]) > 1e-10)) continue;
+            double proj = 0.0;
+            for (int a = 0; a < 6; ++a) proj += W.V[a * 6 + k] * (-g[a]);
+            const double cf = proj / lam[k];
+            for (int a = 0; a < 6; ++a) d[a] += cf * W.V[a * 6 + k];
+          }
+          W.ok = 1;
+        }
+        W.have_basis = 1;
+      }
     }
   }
-  double ev[6], V[36];
-  sym_eigen<6>(H, ev, V);
-  bool neg_def = true;
-  for (int i = 0; i < 6; ++i) neg_def &= (ev[i] < 0.0);
-  double lam[6];
-  if (neg_def) {
-    for (int i = 0; i < 6; ++i) lam[i] = ev[i];
-  } else {
-    for (int i = 0; i < 6; ++i) lam[i] = ev[i] > -1e-3 ? -1e-3 : ev[i];
-  }
-  double max_sv = 0.0;
-  for (int i = 0; i < 6; ++i) max_sv = fmax(max_sv, fabs(lam[i]));
-  if (max_sv < 1e-10) return false;
-  for (int i = 0; i < 6; ++i) d[i] = 0.0;
-  for (int k = 0; k < 6; ++k) {
-    if (!(fabs(lam[k]) > 1e-10)) continue;
-    double proj = 0.0;
-    for (int i = 0; i < 6; ++i) proj += V[i * 6 + k] * (-g[i]);
-    const double c = proj / lam[k];
-    for (int i = 0; i < 6; ++i) d[i] += c * V[i * 6 + k];
-  }
-  return true;
+  __syncthreads();
 }
 
 }  // namespace ndt

@@ -53,7 +53,7 @@ struct DeviceGrid {
   // voxel slots (one per segment, in Morton order)
   uint32_t segments = 0;
   uint32_t num_valid = 0;
-  DevBuf vox, cov, axis, count, coord, valid, table;
+  DevBuf vox, cov, axis, count, coord, valid, table, bitmap;
   GridView view{};
   // host mirrors
   uint32_t* host_scalars = nullptr;  // pinned, 64 words
@@ -85,6 +85,9 @@ struct NewtonConfig {
   int32_t reg_active;  // regularisation enabled and a reference pose set
   int32_t precision;
   int32_t wg_per_slot;
+  // Phase stamps (null = off): workgroup 0 accumulates wall-clock ticks per
+  // phase into phase[0..5] and counts into phase[6..7].
+  unsigned long long* phase;
 };
 
 static_assert(sizeof(ndt_result) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8, "ndt_result layout");

@@ -30,6 +30,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxIter = 256;
 constexpr int kPartialStride = 32;  // doubles per workgroup partial
+static_assert((kBlock / 32) * 32 >= (kBlock / kWave) * kAccSlots, "slot_reduce scratch");
 
 // ---------------------------------------------------------------------------
 // More-Thuente line search as a state machine fed one (score, derivative)
@@ -186,31 +187,126 @@ __device__ inline void point_jacobian(const PoseTables& T, double px, double py,
 
 // Accumulates the derivative contributions of points [first, n) step `stride`.
 // acc: 28 doubles (score, grad, H lower), corr: pair count.
+//
+// Every pair term of compute_derivative_single (derivatives/cpu.rs:139-229)
+// has the form J_i^T X J_j or y . d2T/dp_i dp_j with J the point's 3x6
+// Jacobian, so the pairs of one point are first summed into
+//   s = sum -d1 e_raw,  a = sum e xc,  M = sum e (C^T - d2 xc xc^T)
+// (e = d1 d2 e_raw, xc = x^T C) and the Jacobian is applied once per point:
+//   g_j = a . J_j,  H_ij = J_i^T M J_j + a . d2T/dp_i dp_j   (i >= j).
+// The point-to-plane metric (cpu.rs:246-329) has the same shape with
+// a = sum e d n and M = sum e (1 - d2 d^2) n n^T.
 template <bool kHess>
 __device__ inline void derivative_points(const GridView& g, const NewtonConfig& cfg, const PoseTables& T,
                                          const float* __restrict__ pts, uint32_t n, uint32_t first,
                                          uint32_t stride, double* acc, uint32_t& corr) {
+  const double d1 = cfg.d1, d2 = cfg.d2;
   for (uint32_t i = first; i < n; i += stride) {
     const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
     double t[3];
     t[0] = T.R[0] * px + T.R[1] * py + T.R[2] * pz + T.t[0];
     t[1] = T.R[3] * px + T.R[4] * py + T.R[5] * pz + T.t[1];
     t[2] = T.R[6] * px + T.R[7] * py + T.R[8] * pz + T.t[2];
-    double Jr[3][3], ht[15];
-    point_jacobian(T, px, py, pz, Jr, ht, kHess);
-    for_each_neighbor(g, cfg.search_mode, (float)t[0], (float)t[1], (float)t[2], [&](int32_t s, const float4 m0) {
-      const float4 m1 = g.vox[3 * s + 1];
-      const float4 m2 = g.vox[3 * s + 2];
-      const double x[3] = {t[0] - (double)m0.x, t[1] - (double)m0.y, t[2] - (double)m0.z};
-      corr += 1;
+    double sc = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    double M[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) M[k] = 0.0;
+    uint32_t np = 0;
+    for_each_neighbor(g, cfg.search_mode, (float)t[0], (float)t[1], (float)t[2],
+                      [&](int32_t s, float mx, float my, float mz) {
+      const double x0 = t[0] - (double)mx, x1 = t[1] - (double)my, x2 = t[2] - (double)mz;
+      ++corr;
+      ++np;
       if (cfg.metric == 1) {
-        const double nn[3] = {(double)g.axis[3 * s], (double)g.axis[3 * s + 1], (double)g.axis[3 * s + 2]};
-        pair_p2plane(x, nn, Jr, ht, cfg.d1, cfg.d2, kHess, acc);
+        const double n0 = g.axis[3 * s], n1 = g.axis[3 * s + 1], n2 = g.axis[3 * s + 2];
+        const double d = x0 * n0 + x1 * n1 + x2 * n2;
+        const double dsq = d * d;
+        const double e_raw = exp(-d2 * dsq * 0.5);
+        const double nz = d2 * e_raw;
+        if (!(nz >= 0.0 && nz <= 1.0) || !isfinite(nz)) return;  // counted, contributes zero
+        const double e = d1 * d2 * e_raw;
+        sc += -d1 * e_raw;
+        a0 += e * d * n0;
+        a1 += e * d * n1;
+        a2 += e * d * n2;
+        if (kHess) {
+          const double f = e * (-d2 * dsq + 1.0);
+          const double nn[3] = {n0, n1, n2};
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) M[3 * r + c] += f * (nn[r] * nn[c]);
+        }
       } else {
+        const float4 m0 = g.vox[3 * s];
+        const float4 m1 = g.vox[3 * s + 1];
+        const float4 m2 = g.vox[3 * s + 2];
         const double C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
-        pair_p2d(x, C, Jr, ht, cfg.d1, cfg.d2, kHess, acc);
+        const double xc0 = x0 * C[0] + x1 * C[3] + x2 * C[6];
+        const double xc1 = x0 * C[1] + x1 * C[4] + x2 * C[7];
+        const double xc2 = x0 * C[2] + x1 * C[5] + x2 * C[8];
+        const double xcx = xc0 * x0 + xc1 * x1 + xc2 * x2;
+        const double e_raw = exp(-d2 * xcx * 0.5);
+        const double nz = d2 * e_raw;
+        if (!(nz >= 0.0 && nz <= 1.0) || !isfinite(nz)) return;  // counted, contributes zero
+        const double e = nz * d1;
+        sc += -d1 * e_raw;
+        a0 += e * xc0;
+        a1 += e * xc1;
+        a2 += e * xc2;
+        if (kHess) {
+          const double xc[3] = {xc0, xc1, xc2};
+          const double ed = -d2 * e;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) M[3 * r + c] += e * C[3 * c + r] + ed * (xc[r] * xc[c]);
+        }
       }
     });
+    if (np == 0) continue;
+    double Jr[3][3], ht[15];
+    point_jacobian(T, px, py, pz, Jr, ht, kHess);
+    acc[0] += sc;
+    acc[1] += a0;
+    acc[2] += a1;
+    acc[3] += a2;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[4 + c] += a0 * Jr[0][c] + a1 * Jr[1][c] + a2 * Jr[2][c];
+    if (kHess) {
+      double MJ[3][3];  // M * Jr
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) MJ[r][c] = M[3 * r] * Jr[0][c] + M[3 * r + 1] * Jr[1][c] + M[3 * r + 2] * Jr[2][c];
+      // translation block
+      acc[7 + tri(0, 0)] += M[0];
+      acc[7 + tri(1, 0)] += M[3];
+      acc[7 + tri(1, 1)] += M[4];
+      acc[7 + tri(2, 0)] += M[6];
+      acc[7 + tri(2, 1)] += M[7];
+      acc[7 + tri(2, 2)] += M[8];
+      // rotation rows against translation columns: sum_k Jr[k][i] M[k][j]
+#pragma unroll
+      for (int ri = 0; ri < 3; ++ri)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          acc[7 + tri(3 + ri, j)] += Jr[0][ri] * M[j] + Jr[1][ri] * M[3 + j] + Jr[2][ri] * M[6 + j];
+      // rotation block plus the point-Hessian terms a . d2T/dp_i dp_j
+      const double pa = a1 * ht[0] + a2 * ht[1];
+      const double pb = a1 * ht[2] + a2 * ht[3];
+      const double pc = a1 * ht[4] + a2 * ht[5];
+      const double pd = a0 * ht[6] + a1 * ht[7] + a2 * ht[8];
+      const double pe = a0 * ht[9] + a1 * ht[10] + a2 * ht[11];
+      const double pf = a0 * ht[12] + a1 * ht[13] + a2 * ht[14];
+      const double ph[3][3] = {{pa, pb, pc}, {pb, pd, pe}, {pc, pe, pf}};
+#pragma unroll
+      for (int ri = 0; ri < 3; ++ri)
+#pragma unroll
+        for (int rj = 0; rj <= ri; ++rj)
+          acc[7 + tri(3 + ri, 3 + rj)] +=
+              Jr[0][ri] * MJ[0][rj] + Jr[1][ri] * MJ[1][rj] + Jr[2][ri] * MJ[2][rj] + ph[ri][rj];
+    }
   }
 }
 
@@ -234,8 +330,8 @@ __device__ inline void nvtl_points(const GridView& g, double d1, double d2, cons
     iso_transform(q, tr, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], x);
     double best = -HUGE_VAL;
     bool any = false;
-    for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, const float4 m0) {
-      const double sc = point_score(x, m0, g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
+    for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, float, float, float) {
+      const double sc = point_score(x, g.vox[3 * s], g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
       best = fmax(best, sc);
       any = true;
     });
@@ -249,6 +345,9 @@ __device__ inline void nvtl_points(const GridView& g, double d1, double d2, cons
 // ---------------------------------------------------------------------------
 struct SlotState {
   double pose[6];
+  double grad[6];
+  double delta[6];
+  double score;
   double best_pose[6];
   double dir[6];
   double last_H[36];
@@ -262,40 +361,59 @@ struct SlotState {
   int nhist;
   int status;       // -1 running
   int iterations;
-  int ctrl;         // 0: take a step, 1: stop, 2: line search
+  int ctrl;         // 0: take a step, 1: stop, 2: line search, 3: break to the max-iterations result
+  int need_solve;
   uint32_t last_corr, final_corr;
 };
 
-__device__ inline void slot_barrier(uint32_t* counter, uint32_t target) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(2);
-  }
-  __syncthreads();
-  __threadfence();
-}
-
-// Reduces `count` per-thread values across the slot.  Result in tot[0..count)
-// (LDS), identical in every workgroup of the slot.
+// Reduces `count` (<= 32) per-thread values across the slot.  Result in
+// tot[0..count) (LDS), identical in every workgroup of the slot.
+//
+// With more than one workgroup per slot, thread 0 publishes the workgroup
+// total with agent-scope atomic stores, increments the slot counter with
+// release semantics and polls until every workgroup of the slot has arrived;
+// then every thread acquires the counter and the workgroup sums the partials
+// in a fixed order.  Partials alternate between two
+// buffers so that a fast workgroup writing the next reduction never touches
+// a buffer a slow one may still be reading.
 __device__ inline void slot_reduce(const double* vals, int count, double* lds_scratch, double* wg_tot, double* tot,
-                                   double* partials, uint32_t* counter, uint32_t& gen, int W, int rank) {
+                                   double* partials, uint32_t* counter, uint32_t& gen, int W, int rank,
+                                   unsigned long long* stamp = nullptr) {
   block_sum<kBlock>(vals, count, lds_scratch, wg_tot);
+  if (stamp && threadIdx.x == 0) *stamp = wall_clock64();
   if (W == 1) {
     if (threadIdx.x < (unsigned)count) tot[threadIdx.x] = wg_tot[threadIdx.x];
     __syncthreads();
     return;
   }
   double* buf = partials + (size_t)(gen & 1) * W * kPartialStride;
-  if (threadIdx.x < (unsigned)count) buf[rank * kPartialStride + threadIdx.x] = wg_tot[threadIdx.x];
   ++gen;
-  slot_barrier(counter, gen * (uint32_t)W);
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < count; ++i)
+      __hip_atomic_store(buf + rank * kPartialStride + i, wg_tot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t target = gen * (uint32_t)W;
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    // Poll with relaxed loads (an acquire per poll would invalidate the
+    // caches every time); every thread acquires once, below.
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+  (void)__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  // 8 groups of 32 threads: thread (grp, v) sums value v over ranks grp,
+  // grp+8, ...; then threads < count add the 8 group sums in order.  The
+  // order of additions is fixed, so every workgroup gets the same bits.
+  {
+    const int v = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    double acc = 0.0;
+    if (v < count)
+      for (int r = grp; r < W; r += kBlock / 32) acc += buf[r * kPartialStride + v];
+    lds_scratch[grp * 32 + v] = acc;
+  }
+  __syncthreads();
   if (threadIdx.x < (unsigned)count) {
     double s = 0.0;
-    for (int r = 0; r < W; ++r)
-      s += __hip_atomic_load(buf + r * kPartialStride + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int grp = 0; grp < kBlock / 32; ++grp) s += lds_scratch[grp * 32 + threadIdx.x];
     tot[threadIdx.x] = s;
   }
   __syncthreads();
@@ -351,6 +469,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
                                                         uint32_t* __restrict__ counters) {
   __shared__ PoseTables s_tab;
   __shared__ SlotState S;
+  __shared__ SolveScratch s_solve;
   __shared__ double s_scratch[(kBlock / kWave) * kAccSlots];
   __shared__ double s_wg[kAccSlots];
   __shared__ double s_tot[kAccSlots];
@@ -366,6 +485,15 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   const uint32_t first = rank * kBlock + threadIdx.x;
   const uint32_t stride = (uint32_t)W * kBlock;
   const int max_iter = cfg.max_iter < (uint32_t)kMaxIter ? (int)cfg.max_iter : kMaxIter;
+  // Phase stamps: workgroup 0 thread 0 only.
+  const bool rec = cfg.phase != nullptr && blockIdx.x == 0 && threadIdx.x == 0;
+  // Every workgroup's thread 0 times its own derivative passes (for the
+  // slowest-workgroup figure).
+  const bool rec_all = cfg.phase != nullptr && threadIdx.x == 0;
+  unsigned long long own_deriv = 0, t_own = 0, t_solve0 = 0, t_solve1 = 0, ph_pre = 0, ph_step = 0;
+  unsigned long long ph[6] = {0, 0, 0, 0, 0, 0}, t_iter = 0, t_blk = 0, t_red = 0, t_start = 0;
+  unsigned long long n_ls = 0, n_it = 0;
+  if (rec) t_start = wall_clock64();
 
   if (threadIdx.x == 0) {
     for (int i = 0; i < 6; ++i) {
@@ -380,22 +508,34 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
     S.nhist = 1;
     S.status = -1;
     S.last_corr = 0;
+    s_solve.have_basis = 0;
     make_pose_tables(S.pose, &s_tab);
   }
   __syncthreads();
 
   for (int it = 0; it < max_iter; ++it) {
+    if (rec) t_iter = wall_clock64();
+    if (rec_all) t_own = wall_clock64();
     double acc[kAccSlots];
 #pragma unroll
     for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
     uint32_t corr = 0;
     derivative_points<true>(g, cfg, s_tab, job.pts, job.n, first, stride, acc, corr);
     acc[kAccVals] = (double)corr;
-    slot_reduce(acc, kAccVals + 1, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank);
+    slot_reduce(acc, kAccVals + 1, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank,
+                rec_all ? &t_blk : nullptr);
+    if (rec_all) own_deriv += t_blk - t_own;
+    if (rec) {
+      t_red = wall_clock64();
+      ph[0] += t_blk - t_iter;
+      ph[1] += t_red - t_blk;
+      ++n_it;
+    }
 
     if (threadIdx.x == 0) {
       const uint32_t c = (uint32_t)s_tot[kAccVals];
       S.ctrl = 0;
+      S.need_solve = 0;
       if (c < 10) {  // solver.rs:168-174
         if (it == 0) {
           S.status = NDT_NO_CORRESPONDENCES;
@@ -407,61 +547,70 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
         S.last_corr = c;
         double rs, rg[6], rH[36];
         regularization_terms(cfg, S.pose, c, &rs, rg, rH);
-        const double score = s_tot[0] + rs;
-        double grad[6], H[36];
-        for (int i = 0; i < 6; ++i) grad[i] = s_tot[1 + i] + rg[i];
+        S.score = s_tot[0] + rs;
+        for (int i = 0; i < 6; ++i) S.grad[i] = s_tot[1 + i] + rg[i];
         for (int i = 0; i < 6; ++i)
           for (int j = 0; j <= i; ++j) {
             const double v = s_tot[7 + tri(i, j)];
-            H[i * 6 + j] = v + rH[i * 6 + j];
-            H[j * 6 + i] = v + rH[j * 6 + i];
+            S.last_H[i * 6 + j] = v + rH[i * 6 + j];
+            S.last_H[j * 6 + i] = v + rH[j * 6 + i];
           }
-        for (int i = 0; i < 36; ++i) S.last_H[i] = H[i];
-        if (score > S.best_score) {
-          S.best_score = score;
+        if (S.score > S.best_score) {
+          S.best_score = S.score;
           for (int i = 0; i < 6; ++i) S.best_pose[i] = S.pose[i];
         }
-        double delta[6];
-        if (!newton_step_device(grad, H, delta)) {
-          S.status = NDT_SINGULAR_HESSIAN;
-          for (int i = 0; i < 6; ++i) S.final_pose[i] = S.best_pose[i];
-          S.final_score = S.best_score;
-          S.iterations = it;
+        S.need_solve = 1;
+      }
+    }
+    __syncthreads();
+    if (rec) t_solve0 = wall_clock64();
+    if (S.need_solve) newton_step_wg(S.last_H, S.grad, S.delta, s_solve);
+    if (rec) {
+      t_solve1 = wall_clock64();
+      ph_pre += t_solve0 - t_red;
+      ph_step += t_solve1 - t_solve0;
+    }
+    if (threadIdx.x == 0 && S.need_solve) {
+      const uint32_t c = S.last_corr;
+      if (!s_solve.ok) {  // solver.rs:204-227
+        S.status = NDT_SINGULAR_HESSIAN;
+        for (int i = 0; i < 6; ++i) S.final_pose[i] = S.best_pose[i];
+        S.final_score = S.best_score;
+        S.iterations = it;
+        for (int i = 0; i < 36; ++i) S.final_H[i] = S.last_H[i];
+        S.final_corr = S.last_corr;
+        S.ctrl = 1;
+      } else {
+        double norm = 0.0;
+        for (int i = 0; i < 6; ++i) norm += S.delta[i] * S.delta[i];
+        norm = sqrt(norm);
+        if (norm < cfg.trans_eps) {  // solver.rs:229-253
+          S.status = NDT_CONVERGED;
+          for (int i = 0; i < 6; ++i) S.final_pose[i] = S.pose[i];
+          S.final_score = S.score;
+          S.iterations = it + 1;
           for (int i = 0; i < 36; ++i) S.final_H[i] = S.last_H[i];
-          S.final_corr = S.last_corr;
+          S.final_corr = c;
           S.ctrl = 1;
         } else {
-          double norm = 0.0;
-          for (int i = 0; i < 6; ++i) norm += delta[i] * delta[i];
-          norm = sqrt(norm);
-          if (norm < cfg.trans_eps) {  // solver.rs:229-253
-            S.status = NDT_CONVERGED;
-            for (int i = 0; i < 6; ++i) S.final_pose[i] = S.pose[i];
-            S.final_score = score;
-            S.iterations = it + 1;
-            for (int i = 0; i < 36; ++i) S.final_H[i] = H[i];
-            S.final_corr = c;
-            S.ctrl = 1;
-          } else {
-            double gd = 0.0;
-            for (int i = 0; i < 6; ++i) {
-              S.dir[i] = delta[i] / norm;
-              gd += grad[i] * S.dir[i];
-            }
-            if (gd <= 0.0)
-              for (int i = 0; i < 6; ++i) S.dir[i] = -S.dir[i];
-            if (cfg.use_ls) {
-              double d0 = 0.0;
-              for (int i = 0; i < 6; ++i) d0 += grad[i] * S.dir[i];
-              if (d0 <= 0.0) {
-                S.step = cfg.step_size;
-              } else {
-                mt_begin(S.mt, score, d0, cfg.step_size);
-                S.ctrl = 2;
-              }
+          double gd = 0.0;
+          for (int i = 0; i < 6; ++i) {
+            S.dir[i] = S.delta[i] / norm;
+            gd += S.grad[i] * S.dir[i];
+          }
+          if (gd <= 0.0)
+            for (int i = 0; i < 6; ++i) S.dir[i] = -S.dir[i];
+          if (cfg.use_ls) {
+            double d0 = 0.0;
+            for (int i = 0; i < 6; ++i) d0 += S.grad[i] * S.dir[i];
+            if (d0 <= 0.0) {
+              S.step = cfg.step_size;
             } else {
-              S.step = fmin(norm, cfg.step_size);
+              mt_begin(S.mt, S.score, d0, cfg.step_size);
+              S.ctrl = 2;
             }
+          } else {
+            S.step = fmin(norm, cfg.step_size);
           }
         }
       }
@@ -472,9 +621,11 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
       }
     }
     __syncthreads();
+    if (rec) ph[2] += wall_clock64() - t_red;
     const int ctrl = S.ctrl;
     if (ctrl == 1 || ctrl == 3) break;
     if (ctrl == 2) {
+      const unsigned long long t_ls = rec ? wall_clock64() : 0ull;
       // line_search_with_regularization (solver.rs:640-699): phi(a) = score and
       // directional derivative at pose + a*dir, without the Hessian.
       for (;;) {
@@ -501,8 +652,10 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
           }
         }
         __syncthreads();
+        if (rec) ++n_ls;
         if (S.mt.done) break;
       }
+      if (rec) ph[3] += wall_clock64() - t_ls;
     }
     if (threadIdx.x == 0) {
       for (int i = 0; i < 6; ++i) S.pose[i] = S.pose[i] + S.dir[i] * S.step;  // types.rs:183-197
@@ -526,11 +679,23 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   if (threadIdx.x == 0) quat_from_euler(S.final_pose[3], S.final_pose[4], S.final_pose[5], s_q);
   __syncthreads();
 
+  const unsigned long long t_nv = rec ? wall_clock64() : 0ull;
   double nv_sum = 0.0, nv_cnt = 0.0;
   const bool no_corr = S.status == NDT_NO_CORRESPONDENCES;
   if (!no_corr) nvtl_points(g, cfg.d1, cfg.d2, s_q, S.final_pose, job.pts, job.n, first, stride, nv_sum, nv_cnt);
   double nv[2] = {nv_sum, nv_cnt};
   slot_reduce(nv, 2, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank);
+  if (rec) {
+    const unsigned long long t_end = wall_clock64();
+    ph[4] = t_end - t_nv;
+    ph[5] = t_end - t_start;
+    for (int i = 0; i < 6; ++i) cfg.phase[i] = ph[i];
+    cfg.phase[6] = n_it;
+    cfg.phase[7] = n_ls;
+    cfg.phase[9] = ph_pre;
+    cfg.phase[10] = ph_step;
+  }
+  if (rec_all) atomicMax(&cfg.phase[8], own_deriv);
 
   if (rank == 0 && threadIdx.x == 0) {
     ndt_result r;
@@ -622,8 +787,8 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     if (kind == kScoreNvtl || kind == kScoreNvtlCell27) {
       double best = -HUGE_VAL;
       bool any = false;
-      auto vis = [&](int32_t s, const float4 m0) {
-        best = fmax(best, point_score(x, m0, g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2));
+      auto vis = [&](int32_t s, float, float, float) {
+        best = fmax(best, point_score(x, g.vox[3 * s], g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2));
         any = true;
       };
       if (kind == kScoreNvtl)
@@ -637,8 +802,8 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     } else {
       double ps = 0.0;
       double pc = 0.0;
-      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, const float4 m0) {
-        ps += point_score(x, m0, g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
+      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, float, float, float) {
+        ps += point_score(x, g.vox[3 * s], g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
         pc += 1.0;
       });
       if (kind == kScorePerPoint) {
@@ -669,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, co
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t c = 0;
-  auto vis = [&](int32_t s, const float4) {
+  auto vis = [&](int32_t s, float, float, float) {
     if (c < max_k) idx[(size_t)i * max_k + c] = s;
     ++c;
   };

@@ -40,7 +40,34 @@ __global__ void fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Ordered-int encoding so that atomicMin/atomicMax on int32 keys work.
+// Min/max of the per-point voxel keys: wave shuffles, then LDS across the
+// workgroup, then one atomic per value per workgroup.
+__device__ inline void block_minmax_commit(int32_t mn[3], int32_t mx[3], int32_t* bounds) {
+  __shared__ int32_t s_mn[kBuildBlock / kWave][3], s_mx[kBuildBlock / kWave][3];
+  for (int a = 0; a < 3; ++a) {
+    for (int off = 32; off >= 1; off >>= 1) {
+      mn[a] = min(mn[a], __shfl_xor(mn[a], off, kWave));
+      mx[a] = max(mx[a], __shfl_xor(mx[a], off, kWave));
+    }
+  }
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    for (int a = 0; a < 3; ++a) {
+      s_mn[wave][a] = mn[a];
+      s_mx[wave][a] = mx[a];
+    }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    int32_t a = INT32_MAX, b = INT32_MIN;
+    for (int w = 0; w < kBuildBlock / kWave; ++w) {
+      a = min(a, s_mn[w][threadIdx.x]);
+      b = max(b, s_mx[w][threadIdx.x]);
+    }
+    atomicMin(&bounds[threadIdx.x], a);
+    atomicMax(&bounds[3 + threadIdx.x], b);
+  }
+}
+
 __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restrict__ pts, size_t n, float res,
                                                           int32_t* bounds /* minx,miny,minz,maxx,maxy,maxz */) {
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
@@ -51,18 +78,7 @@ __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restric
       mx[a] = max(mx[a], k);
     }
   }
-  for (int a = 0; a < 3; ++a) {
-    for (int off = 32; off >= 1; off >>= 1) {
-      mn[a] = min(mn[a], __shfl_xor(mn[a], off, kWave));
-      mx[a] = max(mx[a], __shfl_xor(mx[a], off, kWave));
-    }
-  }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&bounds[a], mn[a]);
-      atomicMax(&bounds[3 + a], mx[a]);
-    }
-  }
+  block_minmax_commit(mn, mx, bounds);
 }
 
 // Interleaves the bits of (rx, ry, rz) with widths (bx, by, bz): bit i of each
@@ -410,21 +426,38 @@ __global__ __launch_bounds__(kBuildBlock) void voxel_stats(const float* __restri
 }
 
 __global__ __launch_bounds__(kBuildBlock) void hash_insert(const int32_t* __restrict__ coords,
-                                                           const uint32_t* __restrict__ valid, uint32_t segments,
-                                                           uint4* table, uint32_t mask) {
+                                                           const uint32_t* __restrict__ valid,
+                                                           const float4* __restrict__ vox, uint32_t segments,
+                                                           HashEntry* table, uint32_t mask) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= segments || !valid[s]) return;
   const uint64_t key = pack_coord(coords[3 * s], coords[3 * s + 1], coords[3 * s + 2]);
   uint32_t h = (uint32_t)mix64(key) & mask;
   for (;;) {
-    unsigned long long* slot = reinterpret_cast<unsigned long long*>(&table[h]);
+    unsigned long long* slot = &table[h].key;
     const unsigned long long prev = atomicCAS(slot, (unsigned long long)kEmptyKey, (unsigned long long)key);
     if (prev == kEmptyKey || prev == key) {
-      table[h].z = s;
+      const float4 m = vox[3 * s];
+      table[h].slot = s;
+      table[h].mx = m.x;
+      table[h].my = m.y;
+      table[h].mz = m.z;
       return;
     }
     h = (h + 1) & mask;
   }
+}
+
+// Occupancy bitmap over the padded box: bit (x - ox) + dx*((y - oy) + dy*(z - oz)).
+__global__ __launch_bounds__(kBuildBlock) void bitmap_set(const int32_t* __restrict__ coords,
+                                                          const uint32_t* __restrict__ valid, uint32_t segments,
+                                                          int32_t ox, int32_t oy, int32_t oz, uint32_t dx, uint32_t dy,
+                                                          uint32_t* bitmap) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= segments || !valid[s]) return;
+  const uint64_t lin = (uint64_t)(coords[3 * s] - ox) +
+                       (uint64_t)dx * ((uint64_t)(coords[3 * s + 1] - oy) + (uint64_t)dy * (uint64_t)(coords[3 * s + 2] - oz));
+  atomicOr(&bitmap[lin >> 5], 1u << (lin & 31));
 }
 
 // Bounding box of the valid voxel coordinates (for the lookup's fast reject).
@@ -441,20 +474,9 @@ __global__ __launch_bounds__(kBuildBlock) void valid_bounds(const int32_t* __res
       mx[a] = max(mx[a], coords[3 * s + a]);
     }
   }
-  for (int off = 32; off >= 1; off >>= 1) {
-    c += __shfl_xor(c, off, kWave);
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = min(mn[a], __shfl_xor(mn[a], off, kWave));
-      mx[a] = max(mx[a], __shfl_xor(mx[a], off, kWave));
-    }
-  }
-  if ((threadIdx.x & (kWave - 1)) == 0) {
-    atomicAdd(nvalid, c);
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&bounds[a], mn[a]);
-      atomicMax(&bounds[3 + a], mx[a]);
-    }
-  }
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(nvalid, c);
+  block_minmax_commit(mn, mx, bounds);
 }
 
 __global__ void init_bounds(int32_t* b, uint32_t* extra, int nextra) {
@@ -555,7 +577,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStrea
   int32_t* bounds = reinterpret_cast<int32_t*>(G.scalars.ptr) + 8;
   uint32_t* extra = reinterpret_cast<uint32_t*>(G.scalars.ptr);
   hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra, 8);
-  hipLaunchKernelGGL(key_bounds, dim3(grid_for(n, kBuildBlock, 1024)), dim3(kBuildBlock), 0, st, G.points, n, res,
+  hipLaunchKernelGGL(key_bounds, dim3(grid_for(n, kBuildBlock, 512)), dim3(kBuildBlock), 0, st, G.points, n, res,
                      bounds);
   if ((e = hipMemcpyAsync(G.host_scalars + 8, bounds, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
     return fail(e, "bounds copy");
@@ -616,33 +638,58 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStrea
                      reinterpret_cast<float*>(G.axis.ptr), reinterpret_cast<uint32_t*>(G.count.ptr),
                      reinterpret_cast<int32_t*>(G.coord.ptr), reinterpret_cast<uint32_t*>(G.valid.ptr));
   const uint32_t cap = std::max<uint32_t>(1024, next_pow2(2ull * segs));
-  if ((e = G.table.reserve((size_t)cap * 16)) != hipSuccess) return fail(e, "alloc hash");
-  hipLaunchKernelGGL(fill_u32, dim3(grid_for((size_t)cap * 4, 256, 4096)), dim3(256), 0, st,
-                     reinterpret_cast<uint32_t*>(G.table.ptr), (size_t)cap * 4, 0xFFFFFFFFu);
+  if ((e = G.table.reserve((size_t)cap * sizeof(HashEntry))) != hipSuccess) return fail(e, "alloc hash");
+  hipLaunchKernelGGL(fill_u32, dim3(grid_for((size_t)cap * 8, 256, 4096)), dim3(256), 0, st,
+                     reinterpret_cast<uint32_t*>(G.table.ptr), (size_t)cap * 8, 0xFFFFFFFFu);
   hipLaunchKernelGGL(hash_insert, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
                      reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                     segs, reinterpret_cast<uint4*>(G.table.ptr), cap - 1);
+                     reinterpret_cast<const float4*>(G.vox.ptr), segs, reinterpret_cast<HashEntry*>(G.table.ptr),
+                     cap - 1);
   hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra + 1, 1);
-  hipLaunchKernelGGL(valid_bounds, dim3(grid_for(segs, kBuildBlock, 512)), dim3(kBuildBlock), 0, st,
+  hipLaunchKernelGGL(valid_bounds, dim3(grid_for(segs, kBuildBlock, 256)), dim3(kBuildBlock), 0, st,
                      reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
                      segs, bounds, extra + 1);
   if ((e = hipMemcpyAsync(G.host_scalars, extra, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
     return fail(e, "voxel count copy");
-  (void)hipEventRecord(G.ev_build1, st);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
-  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
   G.num_valid = G.host_scalars[1];
   const int32_t* vb = reinterpret_cast<const int32_t*>(G.host_scalars + 8);
   GridView& v = G.view;
   v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
   v.axis = reinterpret_cast<const float*>(G.axis.ptr);
-  v.table = reinterpret_cast<const uint4*>(G.table.ptr);
+  v.table = reinterpret_cast<const HashEntry*>(G.table.ptr);
   v.table_mask = cap - 1;
   for (int a = 0; a < 3; ++a) {
     v.cmin[a] = vb[a];
     v.cmax[a] = vb[3 + a];
   }
   v.res = res;
+  v.bitmap = nullptr;
+  if (G.num_valid > 0) {
+    // Occupancy bitmap over the box padded by two cells, when it has < 2^32 cells.
+    uint64_t cells = 1;
+    uint32_t dims[3];
+    for (int a = 0; a < 3; ++a) {
+      dims[a] = (uint32_t)((int64_t)v.cmax[a] - v.cmin[a] + 5);
+      cells *= dims[a];
+      v.borg[a] = v.cmin[a] - 2;
+      v.bdim[a] = dims[a];
+    }
+    if (cells < (1ull << 32)) {
+      const size_t words = (size_t)(cells / 32) + 2;
+      if ((e = G.bitmap.reserve(words * 4)) != hipSuccess) return fail(e, "alloc bitmap");
+      hipLaunchKernelGGL(fill_u32, dim3(grid_for(words, 256, 4096)), dim3(256), 0, st,
+                         reinterpret_cast<uint32_t*>(G.bitmap.ptr), words, 0u);
+      hipLaunchKernelGGL(bitmap_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+                         reinterpret_cast<const int32_t*>(G.coord.ptr),
+                         reinterpret_cast<const uint32_t*>(G.valid.ptr), segs, v.borg[0], v.borg[1], v.borg[2],
+                         dims[0], dims[1], reinterpret_cast<uint32_t*>(G.bitmap.ptr));
+      v.bitmap = reinterpret_cast<const uint32_t*>(G.bitmap.ptr);
+    }
+  }
+  (void)hipEventRecord(G.ev_build1, st);
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
   G.n_points = n;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, G.ev_build0, G.ev_build1);

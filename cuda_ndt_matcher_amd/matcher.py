@@ -68,6 +68,7 @@ class NdtScanMatcherConfig:
     search_mode: int = SEARCH_KDTREE
     precision: int = PRECISION_FP64
     workgroups_per_alignment: int = 0
+    profile_phases: bool = False
 
     def to_c(self) -> _lib.NdtConfig:
         c = _lib.NdtConfig()
@@ -86,6 +87,7 @@ class NdtScanMatcherConfig:
         c.search_mode = int(self.search_mode)
         c.precision = int(self.precision)
         c.workgroups_per_alignment = int(self.workgroups_per_alignment)
+        c.profile_phases = int(bool(self.profile_phases))
         return c
 
 
@@ -144,6 +146,9 @@ class NdtScanMatcherBuilder:
 
     def workgroups_per_alignment(self, v: int):
         return self._set(workgroups_per_alignment=v)
+
+    def profile_phases(self, v: bool):
+        return self._set(profile_phases=v)
 
     def build(self) -> "NdtScanMatcher":
         return NdtScanMatcher.with_config(self.config)
@@ -370,6 +375,12 @@ class NdtScanMatcher:
 
     def clear_regularization_pose(self) -> None:
         self._check(self._lib.ndt_hip_clear_regularization_pose(self._h))
+
+    def phase_profile(self) -> dict:
+        """Per-phase device time of the last alignment (profile_phases must be on)."""
+        t = _lib.NdtPhaseProfile()
+        self._check(self._lib.ndt_hip_get_phase_profile(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
 
     def timing(self) -> dict:
         t = _lib.NdtTiming()

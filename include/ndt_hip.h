@@ -83,6 +83,7 @@ typedef struct ndt_config {
   int32_t search_mode;                /* ndt_search_mode, default NDT_SEARCH_KDTREE */
   int32_t precision;                  /* ndt_precision, default NDT_PRECISION_FP64 */
   int32_t workgroups_per_alignment;   /* 0 = automatic; otherwise workgroups cooperating on one alignment */
+  int32_t profile_phases;             /* 1 = time the Newton kernel's phases (see ndt_hip_get_phase_profile) */
 } ndt_config;
 
 /* Convergence status (optimization/types.rs:66-82). */
@@ -118,6 +119,23 @@ typedef struct ndt_timing {
   uint32_t align_workgroups;
   uint32_t align_slots;
 } ndt_timing;
+
+/* Per-phase device time of the last align* call, from wall-clock stamps taken
+ * by workgroup 0 of slot 0 inside the Newton kernel (profile_phases = 1).
+ * Times are totals over the launch in microseconds. */
+typedef struct ndt_phase_profile {
+  float derivative_us;  /* per-point derivative passes (this workgroup's share) + block reduction */
+  float exchange_us;    /* partial exchange and slot barrier (0 with one workgroup per slot) */
+  float solve_us;       /* 6x6 Newton step, convergence test and pose update (thread 0) */
+  float line_search_us; /* More-Thuente evaluations (derivative passes without Hessian) */
+  float nvtl_us;        /* final NVTL pass */
+  float total_us;
+  uint32_t iterations;
+  uint32_t line_search_evaluations;
+  float slowest_derivative_us; /* derivative passes of the slowest workgroup of the launch */
+  float solve_setup_us;        /* part of solve_us before the 6x6 step (thread 0 bookkeeping) */
+  float solve_step_us;         /* part of solve_us in the 6x6 step itself */
+} ndt_phase_profile;
 
 typedef struct ndt_handle ndt_handle;
 
@@ -196,6 +214,7 @@ int ndt_hip_laplace_covariance_xy(const double hess[36], double cov[4]);
 int ndt_hip_laplace_covariance_6x6(const double hess[36], double cov[36]);
 
 int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out);
+int ndt_hip_get_phase_profile(const ndt_handle* h, ndt_phase_profile* out);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,7 @@ def test_struct_layouts():
     from cuda_ndt_matcher_amd import _lib
 
     assert C.sizeof(_lib.NdtResult) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8
-    assert C.sizeof(_lib.NdtConfig) == 80
+    assert C.sizeof(_lib.NdtConfig) == 88
 
 
 def test_config_defaults_match_reference():
