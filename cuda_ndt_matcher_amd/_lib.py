@@ -85,6 +85,8 @@ class NdtPhaseProfile(C.Structure):
         ("slowest_derivative_us", C.c_float),
         ("solve_setup_us", C.c_float),
         ("solve_step_us", C.c_float),
+        ("eigen_solves", C.c_uint32),
+        ("jacobi_sweeps", C.c_uint32),
     ]
 
 
@@ -110,6 +112,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise OSError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C cuda_ndt_matcher_amd/csrc")
+    # PyTorch ships its own libamdhip64.so.7.  Loading torch first makes this
+    # library bind to that same HIP runtime (one runtime per process, so
+    # device pointers and streams from torch are valid here); loading this
+    # library first would pull in a second copy from /opt/rocm under the
+    # same soname, which torch then picks up instead of its own.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

@@ -180,6 +180,8 @@ int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, co
     p.slowest_derivative_us = (float)(ph[8] * us);
     p.solve_setup_us = (float)(ph[9] * us);
     p.solve_step_us = (float)(ph[10] * us);
+    p.eigen_solves = (uint32_t)ph[12];
+    p.jacobi_sweeps = (uint32_t)ph[11];
   }
   return NDT_OK;
 }

@@ -353,7 +353,7 @@ __device__ inline void iso_transform(const double q[4], const double t[3], doubl
 // Accumulator of one derivative pass: score, gradient, lower triangle of the
 // Hessian (21, row-major i >= j), pair count.
 constexpr int kAccVals = 28;  // score + 6 + 21
-constexpr int kAccSlots = 30; // + correspondences + points-with-neighbours/spare (as doubles)
+constexpr int kAccSlots = 32; // + correspondences (as a double), padded to a power of two
 
 __device__ inline int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
 
@@ -365,22 +365,73 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Reduces `count` doubles held by every thread into out[] (valid in thread 0
-// after the call; LDS scratch must hold (BLOCK/64) * count doubles).
-template <int BLOCK>
-__device__ inline void block_sum(const double* vals, int count, double* lds, double* out) {
+// Wave totals of N values per lane (N a power of two, at most 32).  Each
+// halving step keeps half of a lane's values and trades the other half with
+// the lane `o` away, so the wave spends N - 1 + log2(64 / N) shuffles instead
+// of 6 N.  On return v[0] holds, in every lane, the wave total of value
+// lane / (64 / N).
+template <int N>
+__device__ inline void wave_sums(double (&v)[N]) {
+  static_assert(N >= 1 && N <= 32 && (N & (N - 1)) == 0, "N must be a power of two <= 32");
+  constexpr int kSteps = N >= 32 ? 5 : N >= 16 ? 4 : N >= 8 ? 3 : N >= 4 ? 2 : N >= 2 ? 1 : 0;
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const int h = N >> (k + 1), o = (kWave / 2) >> k;
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const double send = up ? v[i] : v[i + h];
+      const double keep = up ? v[i + h] : v[i];
+      v[i] = keep + __shfl_xor(send, o, kWave);
+    }
+  }
+#pragma unroll
+  for (int o = (kWave / 2) >> kSteps; o >= 1; o >>= 1) v[0] += __shfl_xor(v[0], o, kWave);
+}
+
+// Workgroup totals of the first `count` of N values per thread: thread
+// t < count returns total t (fixed summation order; other threads return 0).
+// lds must hold (BLOCK / 64) * N doubles; the caller synchronises before
+// reusing it.
+template <int BLOCK, int N>
+__device__ inline double block_sums(double (&v)[N], int count, double* lds) {
+  wave_sums<N>(v);
+  constexpr int kLanesPer = kWave / N;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  constexpr int kWaves = BLOCK / kWave;
-  for (int i = 0; i < count; ++i) {
-    const double s = wave_sum(vals[i]);
-    if (lane == 0) lds[wave * count + i] = s;
-  }
+  if (lane % kLanesPer == 0 && lane / kLanesPer < count) lds[wave * N + lane / kLanesPer] = v[0];
   __syncthreads();
-  if (threadIdx.x < count) {
-    double s = 0.0;
-    for (int w = 0; w < kWaves; ++w) s += lds[w * count + threadIdx.x];
-    out[threadIdx.x] = s;
+  double s = 0.0;
+  if ((int)threadIdx.x < count)
+#pragma unroll
+    for (int w = 0; w < BLOCK / kWave; ++w) s += lds[w * N + threadIdx.x];
+  return s;
+}
+
+// Workgroup totals of the first `count` values per thread through LDS, for
+// wide accumulators (the shuffle tree above would hold every value twice in
+// registers).  Each thread stores its values as column t of a value-major
+// table; then 8 threads per value sum 32 table entries each (stride 8) and
+// combine with three shuffles.  out[0..count) receives the totals (LDS,
+// synchronised on return).  red must hold count * (BLOCK + 8) doubles.
+template <int BLOCK>
+__device__ inline void block_sums_lds(const double* v, int count, double* red, double* out) {
+  static_assert(BLOCK == 256, "8 threads x 32 entries per value");
+  constexpr int kRow = BLOCK + 8;  // padding spreads the value rows over the banks
+  const int t = threadIdx.x;
+  for (int i = 0; i < count; ++i) red[i * kRow + t] = v[i];
+  __syncthreads();
+  const int val = t >> 3, part = t & 7;
+  double s = 0.0;
+  if (val < count) {
+    const double* row = red + val * kRow + part;
+#pragma unroll
+    for (int k = 0; k < BLOCK / 8; ++k) s += row[8 * k];
   }
+  s += __shfl_xor(s, 1, kWave);
+  s += __shfl_xor(s, 2, kWave);
+  s += __shfl_xor(s, 4, kWave);
+  if (part == 0 && val < count) out[val] = s;
   __syncthreads();
 }
 
@@ -431,30 +482,33 @@ __device__ inline void sym_eigen(const double* a_in, double* ev, double* v) {
 }
 
 // ---------------------------------------------------------------------------
-// Workgroup-cooperative 6x6 Newton step in LDS.
+// Workgroup-cooperative 6x6 Newton step.
 //
 // newton_step_negative_definite (optimization/newton.rs:160-176): when every
 // eigenvalue of H is negative the reference solves H d = -g with an SVD
 // pseudo-inverse (tolerance 1e-10); otherwise it clamps eigenvalues above
-// -1e-3 to -1e-3 and solves with the rebuilt matrix.  Here:
-//  - thread 0 factors -H - 1e-10 I (Cholesky).  Success means every
-//    eigenvalue of H is below -1e-10, so the pseudo-inverse is the plain
-//    inverse and a Cholesky solve of (-H) d = g gives the same step;
-//  - otherwise the workgroup runs a parallel cyclic Jacobi on H: each round
-//    applies 3 disjoint rotations (a 1-factorisation of K6, 5 rounds per
-//    sweep) with one thread per matrix entry, then thread 0 applies the
-//    reference's clamp and pseudo-inverse in the eigenbasis.
+// -1e-3 to -1e-3 and solves with the rebuilt matrix.  Two waves work at once:
+//  - wave 1 (one lane) factors -H - 1e-10 I and -H (Cholesky).  Success means
+//    every eigenvalue of H is below -1e-10, so the pseudo-inverse is the
+//    plain inverse and a Cholesky solve of (-H) d = g gives the same step;
+//  - wave 0 runs a parallel cyclic Jacobi on H with element (i, j) of A and
+//    of the eigenvector matrix V in lane 6 i + j.  Each round applies 3
+//    disjoint rotations (a 1-factorisation of K6, 5 rounds per sweep) with
+//    lane shuffles.  It stops early when wave 1 has found H negative
+//    definite; otherwise the step is formed in the eigenbasis with the
+//    reference's clamp.
+// V is kept as the next call's starting basis (A = V^T H V), which for the
+// slowly changing Hessians of successive Newton iterations is nearly
+// diagonal.  Any orthonormal V is a valid start.
 struct SolveScratch {
-  double A[36];
-  double V[36];   // eigenvectors (columns); kept as the next call's starting basis
+  double V[36];    // eigenvectors (columns), row-major
   double T[36];
   double L[36];
-  double c[6];
-  double o[6];
-  int partner[6];
-  int have_basis; // V holds the eigenvectors of the previous call's matrix
-  int fast;
-  int stop;
+  double dc[6];    // Cholesky step
+  int have_basis;  // V holds the eigenvectors of the previous call's matrix
+  int sweeps;      // Jacobi sweeps run (profiling)
+  int solves;      // calls that completed the eigendecomposition (profiling)
+  int fast;        // set by wave 1 when the Cholesky step exists
   int ok;
 };
 
@@ -475,140 +529,217 @@ __device__ inline bool neg_cholesky6(const double* H, double shift, double* L) {
   return true;
 }
 
-// Round-robin pairing of 6 indices: 5 rounds of 3 disjoint pairs.
-__constant__ static const int kJacobiPairs[5][3][2] = {
-    {{0, 1}, {2, 3}, {4, 5}}, {{0, 2}, {1, 4}, {3, 5}}, {{0, 3}, {1, 5}, {2, 4}},
-    {{0, 4}, {1, 3}, {2, 5}}, {{0, 5}, {1, 2}, {3, 4}}};
+// Round-robin pairing of 6 indices: 5 rounds of 3 disjoint pairs (p < q).
+constexpr int kJacobiPairs[5][3][2] = {{{0, 1}, {2, 3}, {4, 5}}, {{0, 2}, {1, 4}, {3, 5}},
+                                       {{0, 3}, {1, 5}, {2, 4}}, {{0, 4}, {1, 3}, {2, 5}},
+                                       {{0, 5}, {1, 2}, {3, 4}}};
 
 // Orders LDS accesses of the lanes of one wave (no cross-wave barrier).
 __device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
 
-// All threads of the workgroup call this; wave 0 does the work.  H (36, lower
-// triangle read) and g (6) are in LDS; on return W.ok says whether a step
-// exists and d (LDS) holds it.  When W.have_basis is set the Jacobi sweeps
-// start from V^T H V with the previous call's eigenvectors V, which for the
-// slowly changing Hessians of successive Newton iterations is nearly diagonal.
+// Value of x in lane l (l uniform), as a scalar.
+__device__ inline double readlane_f64(double x, int l) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Jacobi rotation (c, s) that zeroes A[p][q] of the symmetric pair block:
+// the smaller of the two angles, |phi| <= pi/4, with tan(2 phi) = 2 A_pq /
+// (A_qq - A_pp).  From r = sqrt(dd^2 + e^2) (dd = A_qq - A_pp, e = 2 A_pq):
+// cos(2 phi) = |dd| / r, c = sqrt((1 + cos(2 phi)) / 2) and
+// s = sin(2 phi) / (2 c) = sgn(dd) e / (2 c r), so two reciprocal square roots
+// and no division.
+__device__ inline void jacobi_rotation(double app, double aqq, double apq, double& c, double& s) {
+  const double dd = aqq - app, e = 2.0 * apq;
+  const double r2 = dd * dd + e * e;
+  const bool rot = apq != 0.0 && r2 > 0.0;
+  const double u = rsqrt(rot ? r2 : 1.0);
+  const double c2 = 0.5 + 0.5 * (fabs(dd) * u);
+  const double ic = rsqrt(c2);
+  c = rot ? c2 * ic : 1.0;
+  s = rot ? 0.5 * ((dd >= 0.0 ? e : -e) * u) * ic : 0.0;
+}
+
+// Per-round pairing tables packed for lookup by a runtime index x in 0..5:
+// partner (3 bits per index), pair number (2 bits) and role (1 bit: x is the
+// q of its pair).
+struct JacobiRoundTables {
+  int partner, pair, is_q;
+};
+__host__ __device__ constexpr JacobiRoundTables jacobi_tables(int r) {
+  JacobiRoundTables t{0, 0, 0};
+  for (int k = 0; k < 3; ++k) {
+    const int p = kJacobiPairs[r][k][0], q = kJacobiPairs[r][k][1];
+    t.partner |= (q << (3 * p)) | (p << (3 * q));
+    t.pair |= (k << (2 * p)) | (k << (2 * q));
+    t.is_q |= 1 << q;
+  }
+  return t;
+}
+
+// One Jacobi round R with element (i, j) of A and of V in lane 6 i + j
+// (lanes 36..63 hold zeros).  Lanes 0..2 compute the round's three rotations;
+// every lane then gathers the four entries of A and two of V that its new
+// value depends on:
+//   A'_ij = c_i c_j A_ij + c_i o_j A_i,mj + o_i c_j A_mi,j + o_i o_j A_mi,mj
+//   V'_ij = c_j V_ij + o_j V_i,mj
+// with m the partner index and o = -s for the p of a pair, +s for the q.
+// The rotated pair entries are set to zero.
+template <int R>
+__device__ inline void jacobi_round(double& aij, double& vij, int i, int j, bool live, int lane) {
+  constexpr JacobiRoundTables T = jacobi_tables(R);
+  constexpr int P0 = kJacobiPairs[R][0][0], Q0 = kJacobiPairs[R][0][1];
+  constexpr int P1 = kJacobiPairs[R][1][0], Q1 = kJacobiPairs[R][1][1];
+  constexpr int P2 = kJacobiPairs[R][2][0], Q2 = kJacobiPairs[R][2][1];
+  // rotation k in lane k
+  const int rp = lane == 0 ? P0 : (lane == 1 ? P1 : P2);
+  const int rq = lane == 0 ? Q0 : (lane == 1 ? Q1 : Q2);
+  const double app = __shfl(aij, rp * 7, kWave);
+  const double aqq = __shfl(aij, rq * 7, kWave);
+  const double apq = __shfl(aij, rp * 6 + rq, kWave);
+  double c, s;
+  jacobi_rotation(app, aqq, apq, c, s);
+  const int mi = (T.partner >> (3 * i)) & 7, mj = (T.partner >> (3 * j)) & 7;
+  const int ki = (T.pair >> (2 * i)) & 3, kj = (T.pair >> (2 * j)) & 3;
+  const double ci = __shfl(c, ki, kWave), si = __shfl(s, ki, kWave);
+  const double cj = __shfl(c, kj, kWave), sj = __shfl(s, kj, kWave);
+  const double oi = ((T.is_q >> i) & 1) ? si : -si;
+  const double oj = ((T.is_q >> j) & 1) ? sj : -sj;
+  const double a_imj = __shfl(aij, i * 6 + mj, kWave);
+  const double a_mij = __shfl(aij, mi * 6 + j, kWave);
+  const double a_mimj = __shfl(aij, mi * 6 + mj, kWave);
+  const double v_imj = __shfl(vij, i * 6 + mj, kWave);
+  const double an = ci * cj * aij + ci * oj * a_imj + oi * cj * a_mij + oi * oj * a_mimj;
+  aij = (!live || mi == j) ? 0.0 : an;
+  vij = live ? cj * vij + oj * v_imj : 0.0;
+}
+
+// All threads of the workgroup call this, after newton_step_reset and a
+// barrier; waves 0 and 1 do the work.  H (36, lower triangle read) and g (6)
+// are in LDS; on return W.ok says whether a step exists and d (LDS) holds it.
 __device__ inline void newton_step_wg(const double* H, const double* g, double* d, SolveScratch& W) {
-  const int lane = threadIdx.x;
-  if (lane < kWave) {
-    if (lane == 0) {
-      W.fast = 0;
-      W.ok = 0;
-      if (neg_cholesky6(H, 1e-10, W.L) && neg_cholesky6(H, 0.0, W.L)) {
-        double y[6];
-        for (int i = 0; i < 6; ++i) {
-          double s = g[i];
-          for (int k = 0; k < i; ++k) s -= W.L[i * 6 + k] * y[k];
-          y[i] = s / W.L[i * 6 + i];
-        }
-        for (int i = 5; i >= 0; --i) {
-          double s = y[i];
-          for (int k = i + 1; k < 6; ++k) s -= W.L[k * 6 + i] * d[k];
-          d[i] = s / W.L[i * 6 + i];
-        }
-        W.fast = 1;
-        W.ok = 1;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+  if (wave == 1) {
+    if (lane == 0 && neg_cholesky6(H, 1e-10, W.L) && neg_cholesky6(H, 0.0, W.L)) {
+      double y[6];
+      for (int i = 0; i < 6; ++i) {
+        double s = g[i];
+        for (int k = 0; k < i; ++k) s -= W.L[i * 6 + k] * y[k];
+        y[i] = s / W.L[i * 6 + i];
       }
+      for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 6; ++k) s -= W.L[k * 6 + i] * W.dc[k];
+        W.dc[i] = s / W.L[i * 6 + i];
+      }
+      __hip_atomic_store(&W.fast, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    wave_lds_fence();
-    if (!W.fast) {
-      const int i = lane / 6, j = lane % 6;
-      if (W.have_basis) {
-        // T = Hs V, A = V^T T (Hs: H symmetrised from its lower triangle)
-        if (lane < 36) {
-          double t = 0.0;
-          for (int k = 0; k < 6; ++k) t += (i >= k ? H[i * 6 + k] : H[k * 6 + i]) * W.V[k * 6 + j];
-          W.T[lane] = t;
-        }
-        wave_lds_fence();
-        double a = 0.0;
-        if (lane < 36)
-          for (int k = 0; k < 6; ++k) a += W.V[k * 6 + i] * W.T[k * 6 + j];
-        wave_lds_fence();
-        if (lane < 36) W.T[lane] = a;
-        wave_lds_fence();
-        if (lane < 36) W.A[lane] = i >= j ? W.T[i * 6 + j] : W.T[j * 6 + i];
-      } else if (lane < 36) {
-        W.A[lane] = i >= j ? H[i * 6 + j] : H[j * 6 + i];
-        W.V[lane] = i == j ? 1.0 : 0.0;
+  } else if (wave == 0) {
+    const bool live = lane < 36;
+    const int i = live ? lane / 6 : 0, j = live ? lane % 6 : 0;
+    double aij, vij;
+    if (W.have_basis) {
+      // T = Hs V, then A = V^T T (Hs: H symmetrised from its lower triangle)
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) t += (i >= k ? H[i * 6 + k] : H[k * 6 + i]) * W.V[k * 6 + j];
+      if (live) W.T[lane] = t;
+      wave_lds_fence();
+      aij = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) aij += W.V[k * 6 + i] * W.T[k * 6 + j];
+      vij = W.V[i * 6 + j];
+    } else {
+      aij = i >= j ? H[i * 6 + j] : H[j * 6 + i];
+      vij = i == j ? 1.0 : 0.0;
+    }
+    if (!live) aij = vij = 0.0;
+    bool fast_exit = false;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+      double off = (i < j) ? aij * aij : 0.0, dg = (live && i == j) ? aij * aij : 0.0;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        off += __shfl_xor(off, o, kWave);
+        dg += __shfl_xor(dg, o, kWave);
+      }
+      off = readlane_f64(off, 0);
+      dg = readlane_f64(dg, 0);
+      if (off == 0.0 || off <= 1e-34 * dg) break;
+      if (__hip_atomic_load(&W.fast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        fast_exit = true;
+        break;
+      }
+      if (lane == 0) ++W.sweeps;
+      jacobi_round<0>(aij, vij, i, j, live, lane);
+      jacobi_round<1>(aij, vij, i, j, live, lane);
+      jacobi_round<2>(aij, vij, i, j, live, lane);
+      jacobi_round<3>(aij, vij, i, j, live, lane);
+      jacobi_round<4>(aij, vij, i, j, live, lane);
+    }
+    if (live) W.V[lane] = vij;
+    if (lane == 0) W.have_basis = 1;
+    if (!fast_exit) {
+      // eigenvalues A_kk (lane 7k), the reference's clamp, and the step in
+      // the eigenbasis: d = sum_k (V_k . (-g)) / lam_k V_k
+      double lam[6];
+      bool neg_def = true;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        lam[k] = readlane_f64(aij, 7 * k);
+        neg_def &= lam[k] < 0.0;
+      }
+      double max_sv = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        lam[k] = (!neg_def && lam[k] > -1e-3) ? -1e-3 : lam[k];
+        max_sv = fmax(max_sv, fabs(lam[k]));
       }
       wave_lds_fence();
-      for (int sweep = 0; sweep < 60; ++sweep) {
-        if (lane == 0) {
-          double off = 0.0, diag = 0.0;
-          for (int a = 0; a < 6; ++a) {
-            diag += W.A[a * 7] * W.A[a * 7];
-            for (int b = a + 1; b < 6; ++b) off += W.A[a * 6 + b] * W.A[a * 6 + b];
-          }
-          W.stop = (off == 0.0 || off <= 1e-34 * diag) ? 1 : 0;
-        }
-        wave_lds_fence();
-        if (W.stop) break;
-        for (int r = 0; r < 5; ++r) {
-          if (lane < 3) {
-            const int p = kJacobiPairs[r][lane][0], q = kJacobiPairs[r][lane][1];
-            const double apq = W.A[p * 6 + q];
-            double c = 1.0, s = 0.0;
-            if (apq != 0.0) {
-              // t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)) with theta = dd / e,
-              // multiplied through by |e|: one sqrt, one division, one rsqrt.
-              const double dd = W.A[q * 7] - W.A[p * 7], e = 2.0 * apq;
-              const double t = (dd >= 0.0 ? e : -e) / (fabs(dd) + sqrt(dd * dd + e * e));
-              c = rsqrt(t * t + 1.0);
-              s = t * c;
-            }
-            W.c[p] = c;
-            W.c[q] = c;
-            W.o[p] = -s;  // G[q][p]
-            W.o[q] = s;   // G[p][q]
-            W.partner[p] = q;
-            W.partner[q] = p;
-          }
-          wave_lds_fence();
-          if (lane < 36) {
-            const int mi = W.partner[i], mj = W.partner[j];
-            const double ci = W.c[i], oi = W.o[i], cj = W.c[j], oj = W.o[j];
-            const double an = (mi == j) ? 0.0
-                                        : ci * cj * W.A[i * 6 + j] + ci * oj * W.A[i * 6 + mj] +
-                                              oi * cj * W.A[mi * 6 + j] + oi * oj * W.A[mi * 6 + mj];
-            const double vn = W.V[i * 6 + j] * cj + W.V[i * 6 + mj] * oj;
-            wave_lds_fence();
-            W.A[lane] = an;
-            W.V[lane] = vn;
-          } else {
-            wave_lds_fence();
-          }
-          wave_lds_fence();
-        }
+      // lane k < 6: coefficient of eigenvector k
+      double cf = 0.0;
+      if (lane < 6) {
+        double proj = 0.0;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) proj += W.V[a * 6 + lane] * (-g[a]);
+        double lk = lam[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k)
+          if (lane == k) lk = lam[k];
+        cf = fabs(lk) > 1e-10 ? proj / lk : 0.0;
       }
-      if (lane == 0) {
-        double ev[6];
-        bool neg_def = true;
-        for (int a = 0; a < 6; ++a) {
-          ev[a] = W.A[a * 7];
-          neg_def &= ev[a] < 0.0;
+      if (lane < 6) W.T[lane] = cf;
+      wave_lds_fence();
+      if (max_sv >= 1e-10) {
+        if (lane < 6) {
+          double di = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) di += W.V[lane * 6 + k] * W.T[k];
+          d[lane] = di;
         }
-        double lam[6], max_sv = 0.0;
-        for (int a = 0; a < 6; ++a) {
-          lam[a] = (!neg_def && ev[a] > -1e-3) ? -1e-3 : ev[a];
-          max_sv = fmax(max_sv, fabs(lam[a]));
-        }
-        if (max_sv >= 1e-10) {
-          for (int a = 0; a < 6; ++a) d[a] = 0.0;
-          for (int k = 0; k < 6; ++k) {
-            if (!(fabs(lam[k]) > 1e-10)) continue;
-            double proj = 0.0;
-            for (int a = 0; a < 6; ++a) proj += W.V[a * 6 + k] * (-g[a]);
-            const double cf = proj / lam[k];
-            for (int a = 0; a < 6; ++a) d[a] += cf * W.V[a * 6 + k];
-          }
-          W.ok = 1;
-        }
-        W.have_basis = 1;
+        if (lane == 0) W.ok = 1;
       }
+      if (lane == 0) ++W.solves;
     }
   }
   __syncthreads();
+  // When the Cholesky step exists the Jacobi may have stopped at a point that
+  // depends on timing; its basis is dropped so that every workgroup of a
+  // slot keeps the same state.
+  if (threadIdx.x < 6 && W.fast) d[threadIdx.x] = W.dc[threadIdx.x];
+  if (threadIdx.x == 0 && W.fast) {
+    W.ok = 1;
+    W.have_basis = 0;
+  }
+  __syncthreads();
+}
+
+// Clears the per-call flags (one thread, before the barrier that precedes
+// newton_step_wg).
+__device__ inline void newton_step_reset(SolveScratch& W) {
+  W.ok = 0;
+  W.fast = 0;
 }
 
 }  // namespace ndt

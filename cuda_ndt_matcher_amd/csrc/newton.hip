@@ -30,7 +30,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxIter = 256;
 constexpr int kPartialStride = 32;  // doubles per workgroup partial
-static_assert((kBlock / 32) * 32 >= (kBlock / kWave) * kAccSlots, "slot_reduce scratch");
+constexpr int kRankGroups = kBlock / 16;  // slot_reduce: 16 column pairs x kRankGroups rank groups
+constexpr int kReduceLds = kRankGroups * kPartialStride;
+static_assert(kReduceLds >= (kBlock / kWave) * kAccSlots, "slot_reduce scratch");
+static_assert(kAccSlots <= kPartialStride, "partial stride");
 
 // ---------------------------------------------------------------------------
 // More-Thuente line search as a state machine fed one (score, derivative)
@@ -202,6 +205,10 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
                                          uint32_t stride, double* acc, uint32_t& corr) {
   const double d1 = cfg.d1, d2 = cfg.d2;
   for (uint32_t i = first; i < n; i += stride) {
+    // Keeps the pose tables in LDS until they are used: a thread usually has
+    // one point, and hoisting the 70 table entries out of this loop would
+    // hold them in registers through the whole pass.
+    asm volatile("" ::: "memory");
     const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
     double t[3];
     t[0] = T.R[0] * px + T.R[1] * py + T.R[2] * pz + T.t[0];
@@ -366,55 +373,91 @@ struct SlotState {
   uint32_t last_corr, final_corr;
 };
 
-// Reduces `count` (<= 32) per-thread values across the slot.  Result in
-// tot[0..count) (LDS), identical in every workgroup of the slot.
+// Slot-wide exchange state, identical in every workgroup of a slot.
+struct SlotSync {
+  double* partials;    // 2 x W x kPartialStride doubles (alternating buffers)
+  uint32_t* counter;   // arrivals, zeroed before the launch
+  uint32_t gen;        // reductions done
+  uint32_t arrivals;   // counter value that completes the current reduction
+  int W;
+  int rank;
+};
+
+// Reduces the first `count` of N per-thread values across the slot.  Result
+// in tot[0..count) (LDS), bit-identical in every workgroup of the slot.
 //
-// With more than one workgroup per slot, thread 0 publishes the workgroup
-// total with agent-scope atomic stores, increments the slot counter with
-// release semantics and polls until every workgroup of the slot has arrived;
-// then every thread acquires the counter and the workgroup sums the partials
-// in a fixed order.  Partials alternate between two
-// buffers so that a fast workgroup writing the next reduction never touches
-// a buffer a slow one may still be reading.
-__device__ inline void slot_reduce(const double* vals, int count, double* lds_scratch, double* wg_tot, double* tot,
-                                   double* partials, uint32_t* counter, uint32_t& gen, int W, int rank,
+// With more than one workgroup per slot, threads t < count publish the
+// workgroup total t and each adds 1 to the slot counter with release
+// semantics; thread 0 polls the counter with relaxed loads until every
+// workgroup's values have arrived and then issues one acquire fence.  The
+// workgroup then reads all partials with independent 16-byte loads (16
+// column pairs x 16 rank groups) and sums them in a fixed order.  Partials
+// alternate between two buffers: a workgroup publishing reduction k+1 has
+// seen every workgroup arrive at reduction k, so none of them still reads
+// the buffer of reduction k-1.
+//
+// N == kAccSlots reduces through the value-major LDS table `red`; smaller N
+// with the wave shuffle tree.
+template <int N>
+__device__ inline void slot_reduce(double (&v)[N], int count, double* lds, double* red, double* tot, SlotSync& sy,
                                    unsigned long long* stamp = nullptr) {
-  block_sum<kBlock>(vals, count, lds_scratch, wg_tot);
-  if (stamp && threadIdx.x == 0) *stamp = wall_clock64();
+  const int t = threadIdx.x;
+  double wg;
+  if constexpr (N == kAccSlots) {
+    block_sums_lds<kBlock>(v, count, red, lds);
+    wg = t < count ? lds[t] : 0.0;
+    __syncthreads();
+  } else {
+    wg = block_sums<kBlock, N>(v, count, lds);
+  }
+  if (stamp && t == 0) *stamp = wall_clock64();
+  const int W = sy.W;
   if (W == 1) {
-    if (threadIdx.x < (unsigned)count) tot[threadIdx.x] = wg_tot[threadIdx.x];
+    if (t < count) tot[t] = wg;
     __syncthreads();
     return;
   }
-  double* buf = partials + (size_t)(gen & 1) * W * kPartialStride;
-  ++gen;
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < count; ++i)
-      __hip_atomic_store(buf + rank * kPartialStride + i, wg_tot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t target = gen * (uint32_t)W;
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    // Poll with relaxed loads (an acquire per poll would invalidate the
-    // caches every time); every thread acquires once, below.
-    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+  double* buf = sy.partials + (size_t)(sy.gen & 1) * W * kPartialStride;
+  ++sy.gen;
+  sy.arrivals += (uint32_t)(W * count);
+  if (t < count) {
+    buf[sy.rank * kPartialStride + t] = wg;
+    __hip_atomic_fetch_add(sy.counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (t == 0) {
+    while (__hip_atomic_load(sy.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sy.arrivals)
       __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  (void)__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  // 8 groups of 32 threads: thread (grp, v) sums value v over ranks grp,
-  // grp+8, ...; then threads < count add the 8 group sums in order.  The
-  // order of additions is fixed, so every workgroup gets the same bits.
   {
-    const int v = threadIdx.x & 31, grp = threadIdx.x >> 5;
-    double acc = 0.0;
-    if (v < count)
-      for (int r = grp; r < W; r += kBlock / 32) acc += buf[r * kPartialStride + v];
-    lds_scratch[grp * 32 + v] = acc;
+    const int pair = t & 15, grp = t >> 4;
+    double s0 = 0.0, s1 = 0.0;
+    if (2 * pair < count) {
+      const double* col = buf + 2 * pair;
+      for (int r0 = grp; r0 < W; r0 += 8 * kRankGroups) {
+        double2 x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int r = r0 + k * kRankGroups;
+          x[k] = r < W ? *reinterpret_cast<const double2*>(col + (size_t)r * kPartialStride) : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          s0 += x[k].x;
+          s1 += x[k].y;
+        }
+      }
+    }
+    lds[grp * kPartialStride + 2 * pair] = s0;
+    lds[grp * kPartialStride + 2 * pair + 1] = s1;
   }
   __syncthreads();
-  if (threadIdx.x < (unsigned)count) {
+  if (t < count) {
     double s = 0.0;
-    for (int grp = 0; grp < kBlock / 32; ++grp) s += lds_scratch[grp * 32 + threadIdx.x];
-    tot[threadIdx.x] = s;
+#pragma unroll
+    for (int g = 0; g < kRankGroups; ++g) s += lds[g * kPartialStride + t];
+    tot[t] = s;
   }
   __syncthreads();
 }
@@ -470,8 +513,8 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   __shared__ PoseTables s_tab;
   __shared__ SlotState S;
   __shared__ SolveScratch s_solve;
-  __shared__ double s_scratch[(kBlock / kWave) * kAccSlots];
-  __shared__ double s_wg[kAccSlots];
+  __shared__ double s_scratch[kReduceLds];
+  __shared__ double s_red[(kAccVals + 1) * (kBlock + 8)];
   __shared__ double s_tot[kAccSlots];
   __shared__ double s_q[4];
 
@@ -479,9 +522,13 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   const int slot = blockIdx.x / W;
   const int rank = blockIdx.x % W;
   const SlotJob job = jobs[slot];
-  double* my_partials = partials + (size_t)slot * 2 * W * kPartialStride;
-  uint32_t* my_counter = counters + slot;
-  uint32_t gen = 0;
+  SlotSync sy;
+  sy.partials = partials + (size_t)slot * 2 * W * kPartialStride;
+  sy.counter = counters + slot;
+  sy.gen = 0;
+  sy.arrivals = 0;
+  sy.W = W;
+  sy.rank = rank;
   const uint32_t first = rank * kBlock + threadIdx.x;
   const uint32_t stride = (uint32_t)W * kBlock;
   const int max_iter = cfg.max_iter < (uint32_t)kMaxIter ? (int)cfg.max_iter : kMaxIter;
@@ -509,6 +556,9 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
     S.status = -1;
     S.last_corr = 0;
     s_solve.have_basis = 0;
+    s_solve.sweeps = 0;
+    s_solve.solves = 0;
+    newton_step_reset(s_solve);
     make_pose_tables(S.pose, &s_tab);
   }
   __syncthreads();
@@ -522,8 +572,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
     uint32_t corr = 0;
     derivative_points<true>(g, cfg, s_tab, job.pts, job.n, first, stride, acc, corr);
     acc[kAccVals] = (double)corr;
-    slot_reduce(acc, kAccVals + 1, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank,
-                rec_all ? &t_blk : nullptr);
+    slot_reduce(acc, kAccVals + 1, s_scratch, s_red, s_tot, sy, rec_all ? &t_blk : nullptr);
     if (rec_all) own_deriv += t_blk - t_own;
     if (rec) {
       t_red = wall_clock64();
@@ -560,6 +609,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
           for (int i = 0; i < 6; ++i) S.best_pose[i] = S.pose[i];
         }
         S.need_solve = 1;
+        newton_step_reset(s_solve);
       }
     }
     __syncthreads();
@@ -629,13 +679,13 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
       // line_search_with_regularization (solver.rs:640-699): phi(a) = score and
       // directional derivative at pose + a*dir, without the Hessian.
       for (;;) {
-        double la[kAccSlots];
+        double la[8];
 #pragma unroll
-        for (int i = 0; i < kAccSlots; ++i) la[i] = 0.0;
+        for (int i = 0; i < 8; ++i) la[i] = 0.0;
         uint32_t lc = 0;
         derivative_points<false>(g, cfg, s_tab, job.pts, job.n, first, stride, la, lc);
         la[7] = (double)lc;
-        slot_reduce(la, 8, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank);
+        slot_reduce(la, 8, s_scratch, s_red, s_tot, sy);
         if (threadIdx.x == 0) {
           double tp[6];
           for (int i = 0; i < 6; ++i) tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
@@ -684,7 +734,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   const bool no_corr = S.status == NDT_NO_CORRESPONDENCES;
   if (!no_corr) nvtl_points(g, cfg.d1, cfg.d2, s_q, S.final_pose, job.pts, job.n, first, stride, nv_sum, nv_cnt);
   double nv[2] = {nv_sum, nv_cnt};
-  slot_reduce(nv, 2, s_scratch, s_wg, s_tot, my_partials, my_counter, gen, W, rank);
+  slot_reduce(nv, 2, s_scratch, s_red, s_tot, sy);
   if (rec) {
     const unsigned long long t_end = wall_clock64();
     ph[4] = t_end - t_nv;
@@ -694,6 +744,8 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
     cfg.phase[7] = n_ls;
     cfg.phase[9] = ph_pre;
     cfg.phase[10] = ph_step;
+    cfg.phase[11] = (unsigned long long)s_solve.sweeps;
+    cfg.phase[12] = (unsigned long long)s_solve.solves;
   }
   if (rec_all) atomicMax(&cfg.phase[8], own_deriv);
 
@@ -737,7 +789,6 @@ __global__ __launch_bounds__(kBlock) void derivative_kernel(GridView g, NewtonCo
                                                             uint32_t n, PoseTables T, int hess,
                                                             double* __restrict__ partials) {
   __shared__ double s_scratch[(kBlock / kWave) * kAccSlots];
-  __shared__ double s_wg[kAccSlots];
   double acc[kAccSlots];
 #pragma unroll
   for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
@@ -748,8 +799,8 @@ __global__ __launch_bounds__(kBlock) void derivative_kernel(GridView g, NewtonCo
   else
     derivative_points<false>(g, cfg, T, pts, n, first, stride, acc, corr);
   acc[kAccVals] = (double)corr;
-  block_sum<kBlock>(acc, kAccVals + 1, s_scratch, s_wg);
-  if (threadIdx.x < kAccVals + 1) partials[blockIdx.x * kPartialStride + threadIdx.x] = s_wg[threadIdx.x];
+  const double tot = block_sums<kBlock, kAccSlots>(acc, kAccVals + 1, s_scratch);
+  if (threadIdx.x < kAccVals + 1) partials[blockIdx.x * kPartialStride + threadIdx.x] = tot;
 }
 
 __global__ void reduce_partials(const double* __restrict__ partials, int blocks, int count, double* __restrict__ out) {
@@ -769,7 +820,6 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
                                                        const double* __restrict__ poses, double* __restrict__ partials,
                                                        double* __restrict__ per_point) {
   __shared__ double s_scratch[(kBlock / kWave) * 2];
-  __shared__ double s_wg[2];
   __shared__ double s_q[4];
   __shared__ double s_t[3];
   const double* pose = poses + 6 * blockIdx.y;
@@ -815,10 +865,10 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     }
   }
   if (kind == kScorePerPoint) return;
-  const double v[2] = {sum, cnt};
-  block_sum<kBlock>(v, 2, s_scratch, s_wg);
+  double v[2] = {sum, cnt};
+  const double tot = block_sums<kBlock, 2>(v, 2, s_scratch);
   if (threadIdx.x < 2)
-    partials[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPartialStride + threadIdx.x] = s_wg[threadIdx.x];
+    partials[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kPartialStride + threadIdx.x] = tot;
 }
 
 __global__ void finish_scores(const double* __restrict__ sums, int m, double* __restrict__ out) {

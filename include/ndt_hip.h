@@ -135,6 +135,8 @@ typedef struct ndt_phase_profile {
   float slowest_derivative_us; /* derivative passes of the slowest workgroup of the launch */
   float solve_setup_us;        /* part of solve_us before the 6x6 step (thread 0 bookkeeping) */
   float solve_step_us;         /* part of solve_us in the 6x6 step itself */
+  uint32_t eigen_solves;       /* steps that needed the eigendecomposition (Hessian not negative definite) */
+  uint32_t jacobi_sweeps;      /* Jacobi sweeps over those steps */
 } ndt_phase_profile;
 
 typedef struct ndt_handle ndt_handle;
