@@ -42,6 +42,7 @@ class NdtConfig(C.Structure):
         ("precision", C.c_int32),
         ("workgroups_per_alignment", C.c_int32),
         ("profile_phases", C.c_int32),
+        ("cell_tables", C.c_int32),
     ]
 
 

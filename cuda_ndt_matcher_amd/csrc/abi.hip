@@ -115,6 +115,9 @@ int choose_wg(const ndt_handle* h, size_t n_max, size_t m) {
   const int cap = std::max(1, h->coop_capacity);
   int per_slot_cap = (int)std::max<size_t>(1, (size_t)cap / std::max<size_t>(1, m));
   if (h->cfg.workgroups_per_alignment > 0) return std::min(h->cfg.workgroups_per_alignment, per_slot_cap);
+  // One point per thread: more workgroups shorten the lookups (each CU's
+  // gather rate bounds them) but lengthen the slot exchange; a second point
+  // per thread doubles the critical path.
   const size_t want = (n_max + 255) / 256;
   return (int)std::max<size_t>(1, std::min<size_t>(want, (size_t)per_slot_cap));
 }
@@ -210,7 +213,8 @@ int run_score(ndt_handle* h, int kind, const float* d_pts, size_t n, const doubl
 
 int finish_target(ndt_handle* h, size_t n) {
   std::string err;
-  const int rc = ndt::build_grid(h->grid, n, h->cfg.resolution, h->cfg.min_points_per_voxel, h->stream, &err);
+  const int rc = ndt::build_grid(h->grid, n, h->cfg.resolution, h->cfg.min_points_per_voxel, h->cfg.cell_tables != 0,
+                                 h->stream, &err);
   h->timing.build_ms = h->grid.build_ms;
   h->timing.sort_ms = h->grid.sort_ms;
   if (rc != 0) {
@@ -261,6 +265,7 @@ void ndt_hip_config_default(ndt_config* c) {
   c->precision = NDT_PRECISION_FP64;
   c->workgroups_per_alignment = 0;
   c->profile_phases = 0;
+  c->cell_tables = 1;
 }
 
 int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
@@ -299,7 +304,7 @@ void ndt_hip_destroy(ndt_handle* h) {
   ndt::DeviceGrid& G = h->grid;
   for (DevBuf* b : {&G.owned_points, &G.sort_keys_a, &G.sort_keys_b, &G.sort_vals_a, &G.sort_vals_b, &G.hist,
                     &G.status, &G.tile_heads, &G.seg_start, &G.scalars, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
-                    &G.valid, &G.table, &G.bitmap, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
+                    &G.valid, &G.table, &G.bitmap, &G.cell_nbr, &G.cell_slot, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
                     &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
   if (G.host_scalars) (void)hipHostFree(G.host_scalars);

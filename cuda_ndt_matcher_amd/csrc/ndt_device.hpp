@@ -12,6 +12,8 @@ namespace ndt {
 constexpr int kWave = 64;
 constexpr uint64_t kEmptyKey = ~0ull;
 constexpr int32_t kCoordBias = 1 << 20;  // voxel coordinates must lie in [-2^20, 2^20)
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint64_t kCellTablesMaxCells = 1ull << 28;
 
 // ---------------------------------------------------------------------------
 // Voxel grid as the kernels see it.  Voxel slots are in Morton-key order; a
@@ -25,12 +27,23 @@ constexpr int32_t kCoordBias = 1 << 20;  // voxel coordinates must lie in [-2^20
 //  - an occupancy bitmap over the voxel box padded by two cells on every
 //    side (one bit per cell) when that box has fewer than 2^32 cells; it
 //    answers "is this cell occupied" for the 3x3x3 block of a query with nine
-//    independent word loads, so the hash is probed only for occupied cells.
+//    independent word loads, so the hash is probed only for occupied cells;
+//  - cell tables over the same padded box when it has at most 2^28 cells:
+//    cell_nbr holds, per cell, the 27-bit occupancy of its 3x3x3 block and
+//    cell_slot the voxel slot of the cell (or kNoSlot).  A query then costs
+//    one load for its block and one per occupied candidate cell, with no
+//    probing.
 struct __align__(32) HashEntry {
   unsigned long long key;
   uint32_t slot;
   float mx, my, mz;
   uint32_t pad[2];
+};
+
+// One voxel's record: {mx,my,mz,C00},{C01,C02,C10,C11},{C12,C20,C21,C22}
+// (mean and inverse covariance, row-major).
+struct VoxelRecord {
+  float4 m0, m1, m2;
 };
 
 struct GridView {
@@ -39,8 +52,10 @@ struct GridView {
   const HashEntry* table;   // capacity is a power of two
   uint32_t table_mask;      // capacity - 1
   const uint32_t* bitmap;   // null when the padded box is too large
-  int32_t borg[3];          // bitmap origin (cmin - 2)
-  uint32_t bdim[3];         // bitmap extent (cmax - cmin + 5)
+  const uint32_t* cell_nbr;   // null when the cell tables are not built
+  const uint32_t* cell_slot;
+  int32_t borg[3];          // padded box origin (cmin - 2)
+  uint32_t bdim[3];         // padded box extent (cmax - cmin + 5)
   int32_t cmin[3];          // bounding box of the occupied voxel coordinates
   int32_t cmax[3];
   float res;
@@ -105,8 +120,9 @@ __device__ inline uint32_t occupancy27(const GridView& g, int32_t kx, int32_t ky
 }
 
 // Visits the voxels of the cells set in `cand` (27-bit mask around k),
-// probing two cells at a time so their table loads overlap.  `accept(e)`
-// decides from the entry (mean) whether to visit; visit(slot, mean).
+// probing two cells at a time so their table loads overlap.
+// accept(mx, my, mz) decides from the voxel mean whether to visit;
+// visit(slot, record).
 template <class A, class F>
 __device__ inline void probe_cells(const GridView& g, int32_t kx, int32_t ky, int32_t kz, uint32_t cand, A&& accept,
                                    F&& visit) {
@@ -138,26 +154,78 @@ __device__ inline void probe_cells(const GridView& g, int32_t kx, int32_t ky, in
       h0 = (h0 + 1) & g.table_mask;
       e0 = g.table[h0];
     }
-    if (e0.key == k0 && accept(e0)) visit((int32_t)e0.slot, e0.mx, e0.my, e0.mz);
+    if (e0.key == k0 && accept(e0.mx, e0.my, e0.mz)) {
+      const VoxelRecord r{g.vox[3 * e0.slot], g.vox[3 * e0.slot + 1], g.vox[3 * e0.slot + 2]};
+      visit((int32_t)e0.slot, r);
+    }
     if (c1 >= 0) {
 #pragma unroll 1
       while (e1.key != k1 && e1.key != kEmptyKey) {
         h1 = (h1 + 1) & g.table_mask;
         e1 = g.table[h1];
       }
-      if (e1.key == k1 && accept(e1)) visit((int32_t)e1.slot, e1.mx, e1.my, e1.mz);
+      if (e1.key == k1 && accept(e1.mx, e1.my, e1.mz)) {
+        const VoxelRecord r{g.vox[3 * e1.slot], g.vox[3 * e1.slot + 1], g.vox[3 * e1.slot + 2]};
+        visit((int32_t)e1.slot, r);
+      }
     }
   }
 }
 
-// Block mask restricted to cells that can exist: all 27 when the centre's
-// block lies in the padded box, and the bitmap's answer when there is one.
-__device__ inline bool block_mask(const GridView& g, int32_t kx, int32_t ky, int32_t kz, uint32_t& mask) {
+// Index of cell (x, y, z) in the padded box (x fastest).
+__device__ inline uint32_t box_index(const GridView& g, int32_t x, int32_t y, int32_t z) {
+  return (uint32_t)(x - g.borg[0]) + g.bdim[0] * ((uint32_t)(y - g.borg[1]) + g.bdim[1] * (uint32_t)(z - g.borg[2]));
+}
+
+// Cell-table visit of the candidate cells around the cell with box index
+// `lin`: four candidates per batch, so their slot loads and then their mean
+// loads are in flight together.  Candidates are visited in increasing cell
+// order, as in probe_cells.
+template <class A, class F>
+__device__ inline void probe_dense(const GridView& g, uint32_t lin, uint32_t cand, A&& accept, F&& visit) {
+  const int32_t sy = (int32_t)g.bdim[0], sz = (int32_t)(g.bdim[0] * g.bdim[1]);
+#pragma unroll 1
+  while (cand) {
+    uint32_t sl[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      sl[b] = kNoSlot;
+      if (cand) {
+        const int c = __ffs(cand) - 1;
+        cand &= cand - 1;
+        const int32_t off = (c / 9 - 1) + (c / 3 % 3 - 1) * sy + (c % 3 - 1) * sz;
+        sl[b] = g.cell_slot[(uint32_t)((int32_t)lin + off)];
+      }
+    }
+    // Whole 48-byte records, so the visit does not wait for a third
+    // dependent round trip (unused parts are dropped by the compiler).
+    VoxelRecord r[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (sl[b] != kNoSlot) r[b] = VoxelRecord{g.vox[3 * sl[b]], g.vox[3 * sl[b] + 1], g.vox[3 * sl[b] + 2]};
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (sl[b] != kNoSlot && accept(r[b].m0.x, r[b].m0.y, r[b].m0.z)) visit((int32_t)sl[b], r[b]);
+  }
+}
+
+// Visits the voxels of the cells in `mask` (27-bit, around k) that exist,
+// through the cell tables when they are built and otherwise through the
+// bitmap and the hash.  Nothing is visited when no occupied cell lies within
+// one cell of k.
+template <class A, class F>
+__device__ inline void probe_block(const GridView& g, int32_t kx, int32_t ky, int32_t kz, uint32_t mask, A&& accept,
+                                   F&& visit) {
   if (kx < g.cmin[0] - 1 || kx > g.cmax[0] + 1 || ky < g.cmin[1] - 1 || ky > g.cmax[1] + 1 ||
       kz < g.cmin[2] - 1 || kz > g.cmax[2] + 1)
-    return false;  // no occupied cell within one of k
+    return;
+  if (g.cell_slot) {
+    const uint32_t lin = box_index(g, kx, ky, kz);
+    probe_dense(g, lin, mask & g.cell_nbr[lin], accept, visit);
+    return;
+  }
   if (g.bitmap) mask &= occupancy27(g, kx, ky, kz);
-  return mask != 0;
+  if (mask) probe_cells(g, kx, ky, kz, mask, accept, visit);
 }
 
 // Rust `f32 as i32` / `f64 as i32`: saturating, NaN -> 0.
@@ -191,7 +259,7 @@ __device__ inline float dist2_f32(float qx, float qy, float qz, float mx, float 
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour enumeration.  visit(slot, mx, my, mz) is called for each
+// Neighbour enumeration.  visit(slot, record) is called for each
 // neighbouring voxel.
 //
 // Radius mode: every voxel whose f32 mean is within `res` of q (strict <).
@@ -205,24 +273,31 @@ __device__ inline void for_each_radius(const GridView& g, float qx, float qy, fl
   const int32_t kx = sat_i32d(floor((double)qx / res));
   const int32_t ky = sat_i32d(floor((double)qy / res));
   const int32_t kz = sat_i32d(floor((double)qz / res));
-  // Distance from q to the low/high faces of its own cell, per axis.
-  const double lox = (double)qx - kx * res, hix = (kx + 1) * res - (double)qx;
-  const double loy = (double)qy - ky * res, hiy = (ky + 1) * res - (double)qy;
-  const double loz = (double)qz - kz * res, hiz = (kz + 1) * res - (double)qz;
+  // Squared distance from q to the low/high faces of its own cell, per axis
+  // (index 0: the cell below, 1: own cell, 2: the cell above).
+  double sx[3], sy[3], sz[3];
+  {
+    const double lx = (double)qx - kx * res, hx = (kx + 1) * res - (double)qx;
+    const double ly = (double)qy - ky * res, hy = (ky + 1) * res - (double)qy;
+    const double lz = (double)qz - kz * res, hz = (kz + 1) * res - (double)qz;
+    sx[0] = lx * lx; sx[1] = 0.0; sx[2] = hx * hx;
+    sy[0] = ly * ly; sy[1] = 0.0; sy[2] = hy * hy;
+    sz[0] = lz * lz; sz[1] = 0.0; sz[2] = hz * hz;
+  }
   const double lim = res * res * (1.0 + 1e-6);
   uint32_t mask = 0;
 #pragma unroll
-  for (int c = 0; c < 27; ++c) {
-    const int dx = c / 9 - 1, dy = (c / 3) % 3 - 1, dz = c % 3 - 1;
-    const double ex = dx < 0 ? lox : (dx > 0 ? hix : 0.0);
-    const double ey = dy < 0 ? loy : (dy > 0 ? hiy : 0.0);
-    const double ez = dz < 0 ? loz : (dz > 0 ? hiz : 0.0);
-    if (ex * ex + ey * ey + ez * ez <= lim) mask |= 1u << c;
-  }
-  if (!block_mask(g, kx, ky, kz, mask)) return;
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const double sxy = sx[a] + sy[b];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (sxy + sz[c] <= lim) mask |= 1u << (9 * a + 3 * b + c);
+    }
   const float r2 = __fmul_rn(g.res, g.res);
-  probe_cells(
-      g, kx, ky, kz, mask, [&](const HashEntry& e) { return dist2_f32(qx, qy, qz, e.mx, e.my, e.mz) < r2; },
+  probe_block(
+      g, kx, ky, kz, mask, [&](float mx, float my, float mz) { return dist2_f32(qx, qy, qz, mx, my, mz) < r2; },
       visit);
 }
 
@@ -240,9 +315,7 @@ __host__ __device__ constexpr uint32_t direct_mask(int mode) {
 template <class F>
 __device__ inline void for_each_direct(const GridView& g, int mode, float qx, float qy, float qz, F&& visit) {
   const int32_t kx = voxel_key(qx, g.res), ky = voxel_key(qy, g.res), kz = voxel_key(qz, g.res);
-  uint32_t mask = direct_mask(mode);
-  if (!block_mask(g, kx, ky, kz, mask)) return;
-  probe_cells(g, kx, ky, kz, mask, [](const HashEntry&) { return true; }, visit);
+  probe_block(g, kx, ky, kz, direct_mask(mode), [](float, float, float) { return true; }, visit);
 }
 
 template <class F>
@@ -259,9 +332,7 @@ template <class F>
 __device__ inline void for_each_cell27(const GridView& g, double x, double y, double z, F&& visit) {
   const double inv = 1.0 / (double)g.res;
   const int32_t vx = sat_i32d(floor(x * inv)), vy = sat_i32d(floor(y * inv)), vz = sat_i32d(floor(z * inv));
-  uint32_t mask = (1u << 27) - 1;
-  if (!block_mask(g, vx, vy, vz, mask)) return;
-  probe_cells(g, vx, vy, vz, mask, [](const HashEntry&) { return true; }, visit);
+  probe_block(g, vx, vy, vz, (1u << 27) - 1, [](float, float, float) { return true; }, visit);
 }
 
 // ---------------------------------------------------------------------------

@@ -53,7 +53,7 @@ struct DeviceGrid {
   // voxel slots (one per segment, in Morton order)
   uint32_t segments = 0;
   uint32_t num_valid = 0;
-  DevBuf vox, cov, axis, count, coord, valid, table, bitmap;
+  DevBuf vox, cov, axis, count, coord, valid, table, bitmap, cell_nbr, cell_slot;
   GridView view{};
   // host mirrors
   uint32_t* host_scalars = nullptr;  // pinned, 64 words
@@ -62,7 +62,8 @@ struct DeviceGrid {
   bool ready = false;
 };
 
-int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStream_t st, std::string* err);
+int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cell_tables, hipStream_t st,
+               std::string* err);
 
 // One alignment request inside a persistent launch.
 struct SlotJob {

@@ -19,6 +19,7 @@
 // transform (scoring/nvtl.rs:211-268) and rank 0 writes the result.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "ndt_device.hpp"
@@ -188,7 +189,8 @@ __device__ inline void point_jacobian(const PoseTables& T, double px, double py,
   }
 }
 
-// Accumulates the derivative contributions of points [first, n) step `stride`.
+// Accumulates the derivative contributions of points first, first + stride,
+// ... below n.
 // acc: 28 doubles (score, grad, H lower), corr: pair count.
 //
 // Every pair term of compute_derivative_single (derivatives/cpu.rs:139-229)
@@ -220,8 +222,8 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
     for (int k = 0; k < 9; ++k) M[k] = 0.0;
     uint32_t np = 0;
     for_each_neighbor(g, cfg.search_mode, (float)t[0], (float)t[1], (float)t[2],
-                      [&](int32_t s, float mx, float my, float mz) {
-      const double x0 = t[0] - (double)mx, x1 = t[1] - (double)my, x2 = t[2] - (double)mz;
+                      [&](int32_t s, const VoxelRecord& rec) {
+      const double x0 = t[0] - (double)rec.m0.x, x1 = t[1] - (double)rec.m0.y, x2 = t[2] - (double)rec.m0.z;
       ++corr;
       ++np;
       if (cfg.metric == 1) {
@@ -245,9 +247,7 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
             for (int c = 0; c < 3; ++c) M[3 * r + c] += f * (nn[r] * nn[c]);
         }
       } else {
-        const float4 m0 = g.vox[3 * s];
-        const float4 m1 = g.vox[3 * s + 1];
-        const float4 m2 = g.vox[3 * s + 2];
+        const float4 m0 = rec.m0, m1 = rec.m1, m2 = rec.m2;
         const double C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
         const double xc0 = x0 * C[0] + x1 * C[3] + x2 * C[6];
         const double xc1 = x0 * C[1] + x1 * C[4] + x2 * C[7];
@@ -337,8 +337,8 @@ __device__ inline void nvtl_points(const GridView& g, double d1, double d2, cons
     iso_transform(q, tr, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], x);
     double best = -HUGE_VAL;
     bool any = false;
-    for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, float, float, float) {
-      const double sc = point_score(x, g.vox[3 * s], g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
+    for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, const VoxelRecord& r) {
+      const double sc = point_score(x, r.m0, r.m1, r.m2, d1, d2);
       best = fmax(best, sc);
       any = true;
     });
@@ -374,30 +374,29 @@ struct SlotState {
 };
 
 // Slot-wide exchange state, identical in every workgroup of a slot.
+//
+// Arrivals are counted in two levels so that no counter takes more than
+// kArrivalGroup + a few atomics per reduction: each workgroup adds to the
+// counter of its group of kArrivalGroup ranks, and the lane that completes a
+// group adds 1 to the slot's top counter, which every workgroup polls.
+// Counters sit in separate 128-byte lines.
+constexpr int kArrivalGroup = 16;
+constexpr int kCounterStride = 32;  // uint32 per counter line
 struct SlotSync {
-  double* partials;    // 2 x W x kPartialStride doubles (alternating buffers)
-  uint32_t* counter;   // arrivals, zeroed before the launch
-  uint32_t gen;        // reductions done
-  uint32_t arrivals;   // counter value that completes the current reduction
+  double* partials;         // 2 x W x kPartialStride doubles (alternating buffers)
+  uint32_t* top;            // slot counter: completed groups
+  uint32_t* group;          // this rank's group counter: arrivals
+  uint32_t group_size;      // workgroups in this rank's group
+  uint32_t groups;          // groups in the slot
+  uint32_t gen;             // reductions done
+  uint32_t group_arrivals;  // group counter value that completes the current reduction
+  uint32_t top_arrivals;    // top counter value that completes the current reduction
   int W;
   int rank;
 };
 
-// Reduces the first `count` of N per-thread values across the slot.  Result
-// in tot[0..count) (LDS), bit-identical in every workgroup of the slot.
-//
-// With more than one workgroup per slot, threads t < count publish the
-// workgroup total t and each adds 1 to the slot counter with release
-// semantics; thread 0 polls the counter with relaxed loads until every
-// workgroup's values have arrived and then issues one acquire fence.  The
-// workgroup then reads all partials with independent 16-byte loads (16
-// column pairs x 16 rank groups) and sums them in a fixed order.  Partials
-// alternate between two buffers: a workgroup publishing reduction k+1 has
-// seen every workgroup arrive at reduction k, so none of them still reads
-// the buffer of reduction k-1.
-//
-// N == kAccSlots reduces through the value-major LDS table `red`; smaller N
-// with the wave shuffle tree.
+__host__ __device__ constexpr int counter_lines(int W) { return 1 + (W + kArrivalGroup - 1) / kArrivalGroup; }
+
 template <int N>
 __device__ inline void slot_reduce(double (&v)[N], int count, double* lds, double* red, double* tot, SlotSync& sy,
                                    unsigned long long* stamp = nullptr) {
@@ -419,13 +418,18 @@ __device__ inline void slot_reduce(double (&v)[N], int count, double* lds, doubl
   }
   double* buf = sy.partials + (size_t)(sy.gen & 1) * W * kPartialStride;
   ++sy.gen;
-  sy.arrivals += (uint32_t)(W * count);
+  sy.group_arrivals += (uint32_t)count * sy.group_size;
+  sy.top_arrivals += sy.groups;
   if (t < count) {
     buf[sy.rank * kPartialStride + t] = wg;
-    __hip_atomic_fetch_add(sy.counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t old = __hip_atomic_fetch_add(sy.group, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1 == sy.group_arrivals) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __hip_atomic_fetch_add(sy.top, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   if (t == 0) {
-    while (__hip_atomic_load(sy.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sy.arrivals)
+    while (__hip_atomic_load(sy.top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sy.top_arrivals)
       __builtin_amdgcn_s_sleep(1);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
@@ -524,13 +528,26 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   const SlotJob job = jobs[slot];
   SlotSync sy;
   sy.partials = partials + (size_t)slot * 2 * W * kPartialStride;
-  sy.counter = counters + slot;
+  {
+    uint32_t* lines = counters + (size_t)slot * counter_lines(W) * kCounterStride;
+    sy.top = lines;
+    sy.group = lines + (size_t)(1 + rank / kArrivalGroup) * kCounterStride;
+    sy.group_size = (uint32_t)min(kArrivalGroup, W - (rank / kArrivalGroup) * kArrivalGroup);
+    sy.groups = (uint32_t)((W + kArrivalGroup - 1) / kArrivalGroup);
+  }
   sy.gen = 0;
-  sy.arrivals = 0;
+  sy.group_arrivals = 0;
+  sy.top_arrivals = 0;
   sy.W = W;
   sy.rank = rank;
-  const uint32_t first = rank * kBlock + threadIdx.x;
-  const uint32_t stride = (uint32_t)W * kBlock;
+  // Workgroup `rank` owns points [begin, last); its thread t takes local
+  // index (t % 64) * 4 + t / 64 and then steps by the block size, so a short
+  // range is shared by all four waves (more loads in flight per CU).
+  const uint32_t per_wg = (job.n + (uint32_t)W - 1) / (uint32_t)W;
+  const uint32_t begin = min(job.n, (uint32_t)rank * per_wg);
+  const uint32_t last = min(job.n, begin + per_wg);
+  const uint32_t first = begin + (threadIdx.x % kWave) * (kBlock / kWave) + threadIdx.x / kWave;
+  const uint32_t stride = kBlock;
   const int max_iter = cfg.max_iter < (uint32_t)kMaxIter ? (int)cfg.max_iter : kMaxIter;
   // Phase stamps: workgroup 0 thread 0 only.
   const bool rec = cfg.phase != nullptr && blockIdx.x == 0 && threadIdx.x == 0;
@@ -570,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 #pragma unroll
     for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
     uint32_t corr = 0;
-    derivative_points<true>(g, cfg, s_tab, job.pts, job.n, first, stride, acc, corr);
+    derivative_points<true>(g, cfg, s_tab, job.pts, last, first, stride, acc, corr);
     acc[kAccVals] = (double)corr;
     slot_reduce(acc, kAccVals + 1, s_scratch, s_red, s_tot, sy, rec_all ? &t_blk : nullptr);
     if (rec_all) own_deriv += t_blk - t_own;
@@ -683,7 +700,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 #pragma unroll
         for (int i = 0; i < 8; ++i) la[i] = 0.0;
         uint32_t lc = 0;
-        derivative_points<false>(g, cfg, s_tab, job.pts, job.n, first, stride, la, lc);
+        derivative_points<false>(g, cfg, s_tab, job.pts, last, first, stride, la, lc);
         la[7] = (double)lc;
         slot_reduce(la, 8, s_scratch, s_red, s_tot, sy);
         if (threadIdx.x == 0) {
@@ -732,7 +749,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   const unsigned long long t_nv = rec ? wall_clock64() : 0ull;
   double nv_sum = 0.0, nv_cnt = 0.0;
   const bool no_corr = S.status == NDT_NO_CORRESPONDENCES;
-  if (!no_corr) nvtl_points(g, cfg.d1, cfg.d2, s_q, S.final_pose, job.pts, job.n, first, stride, nv_sum, nv_cnt);
+  if (!no_corr) nvtl_points(g, cfg.d1, cfg.d2, s_q, S.final_pose, job.pts, last, first, stride, nv_sum, nv_cnt);
   double nv[2] = {nv_sum, nv_cnt};
   slot_reduce(nv, 2, s_scratch, s_red, s_tot, sy);
   if (rec) {
@@ -837,8 +854,8 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     if (kind == kScoreNvtl || kind == kScoreNvtlCell27) {
       double best = -HUGE_VAL;
       bool any = false;
-      auto vis = [&](int32_t s, float, float, float) {
-        best = fmax(best, point_score(x, g.vox[3 * s], g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2));
+      auto vis = [&](int32_t, const VoxelRecord& r) {
+        best = fmax(best, point_score(x, r.m0, r.m1, r.m2, d1, d2));
         any = true;
       };
       if (kind == kScoreNvtl)
@@ -852,8 +869,8 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     } else {
       double ps = 0.0;
       double pc = 0.0;
-      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t s, float, float, float) {
-        ps += point_score(x, g.vox[3 * s], g.vox[3 * s + 1], g.vox[3 * s + 2], d1, d2);
+      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, const VoxelRecord& r) {
+        ps += point_score(x, r.m0, r.m1, r.m2, d1, d2);
         pc += 1.0;
       });
       if (kind == kScorePerPoint) {
@@ -884,7 +901,7 @@ __global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, co
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t c = 0;
-  auto vis = [&](int32_t s, float, float, float) {
+  auto vis = [&](int32_t s, const VoxelRecord&) {
     if (c < max_k) idx[(size_t)i * max_k + c] = s;
     ++c;
   };
@@ -900,7 +917,7 @@ __global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, co
 
 size_t newton_workspace_bytes(int slots, int wg_per_slot) {
   const size_t partial = (size_t)slots * 2 * wg_per_slot * kPartialStride * sizeof(double);
-  return partial + (size_t)slots * sizeof(uint32_t) + 256;
+  return partial + (size_t)slots * counter_lines(wg_per_slot) * kCounterStride * sizeof(uint32_t) + 256;
 }
 
 int newton_max_coresident_workgroups() {
@@ -919,7 +936,8 @@ hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJ
   double* partials = reinterpret_cast<double*>(ws);
   uint32_t* counters =
       reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
-  hipLaunchKernelGGL(zero_u32, dim3((slots + 255) / 256), dim3(256), 0, st, counters, slots);
+  const int ncounters = slots * counter_lines(W) * kCounterStride;
+  hipLaunchKernelGGL(zero_u32, dim3(std::min(1024, (ncounters + 255) / 256)), dim3(256), 0, st, counters, ncounters);
   const dim3 grid((unsigned)(slots * W)), block(kBlock);
   if (cooperative) {
     GridView gg = g;

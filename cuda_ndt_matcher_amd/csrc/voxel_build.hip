@@ -460,6 +460,28 @@ __global__ __launch_bounds__(kBuildBlock) void bitmap_set(const int32_t* __restr
   atomicOr(&bitmap[lin >> 5], 1u << (lin & 31));
 }
 
+// Cell tables: slot of every valid voxel at its cell ...
+__global__ __launch_bounds__(kBuildBlock) void cell_slot_set(const int32_t* __restrict__ coords,
+                                                             const uint32_t* __restrict__ valid, uint32_t segments,
+                                                             GridView v, uint32_t* cell_slot) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= segments || !valid[s]) return;
+  cell_slot[box_index(v, coords[3 * s], coords[3 * s + 1], coords[3 * s + 2])] = s;
+}
+
+// ... and the 27-bit block occupancy of every cell whose block lies inside
+// the padded box (zero on the outer layer, where no query block is centred).
+__global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t cells, uint32_t* cell_nbr) {
+  for (uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x; lin < cells; lin += gridDim.x * blockDim.x) {
+    const uint32_t x = lin % v.bdim[0], yz = lin / v.bdim[0];
+    const uint32_t y = yz % v.bdim[1], z = yz / v.bdim[1];
+    uint32_t m = 0;
+    if (x >= 1 && x + 2 <= v.bdim[0] && y >= 1 && y + 2 <= v.bdim[1] && z >= 1 && z + 2 <= v.bdim[2])
+      m = occupancy27(v, (int32_t)x + v.borg[0], (int32_t)y + v.borg[1], (int32_t)z + v.borg[2]);
+    cell_nbr[lin] = m;
+  }
+}
+
 // Bounding box of the valid voxel coordinates (for the lookup's fast reject).
 __global__ __launch_bounds__(kBuildBlock) void valid_bounds(const int32_t* __restrict__ coords,
                                                             const uint32_t* __restrict__ valid, uint32_t segments,
@@ -558,7 +580,8 @@ uint32_t next_pow2(uint64_t v) {
 
 // Builds the grid from G.points (device, n points).  Returns a status code and
 // fills G's voxel arrays, hash and views.
-int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStream_t st, std::string* err) {
+int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cell_tables, hipStream_t st,
+               std::string* err) {
   auto fail = [&](hipError_t e, const char* what) {
     *err = std::string(what) + ": " + hipGetErrorString(e);
     return -5;
@@ -665,6 +688,8 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStrea
   }
   v.res = res;
   v.bitmap = nullptr;
+  v.cell_nbr = nullptr;
+  v.cell_slot = nullptr;
   if (G.num_valid > 0) {
     // Occupancy bitmap over the box padded by two cells, when it has < 2^32 cells.
     uint64_t cells = 1;
@@ -685,6 +710,21 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, hipStrea
                          reinterpret_cast<const uint32_t*>(G.valid.ptr), segs, v.borg[0], v.borg[1], v.borg[2],
                          dims[0], dims[1], reinterpret_cast<uint32_t*>(G.bitmap.ptr));
       v.bitmap = reinterpret_cast<const uint32_t*>(G.bitmap.ptr);
+      if (cell_tables && cells <= kCellTablesMaxCells) {
+        const uint32_t nc = (uint32_t)cells;
+        if ((e = G.cell_nbr.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
+        if ((e = G.cell_slot.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
+        hipLaunchKernelGGL(fill_u32, dim3(grid_for(nc, 256, 4096)), dim3(256), 0, st,
+                           reinterpret_cast<uint32_t*>(G.cell_slot.ptr), (size_t)nc, kNoSlot);
+        hipLaunchKernelGGL(cell_slot_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+                           reinterpret_cast<const int32_t*>(G.coord.ptr),
+                           reinterpret_cast<const uint32_t*>(G.valid.ptr), segs, v,
+                           reinterpret_cast<uint32_t*>(G.cell_slot.ptr));
+        hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for(nc, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st, v, nc,
+                           reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
+        v.cell_nbr = reinterpret_cast<const uint32_t*>(G.cell_nbr.ptr);
+        v.cell_slot = reinterpret_cast<const uint32_t*>(G.cell_slot.ptr);
+      }
     }
   }
   (void)hipEventRecord(G.ev_build1, st);

@@ -69,6 +69,7 @@ class NdtScanMatcherConfig:
     precision: int = PRECISION_FP64
     workgroups_per_alignment: int = 0
     profile_phases: bool = False
+    cell_tables: bool = True
 
     def to_c(self) -> _lib.NdtConfig:
         c = _lib.NdtConfig()
@@ -88,6 +89,7 @@ class NdtScanMatcherConfig:
         c.precision = int(self.precision)
         c.workgroups_per_alignment = int(self.workgroups_per_alignment)
         c.profile_phases = int(bool(self.profile_phases))
+        c.cell_tables = int(bool(self.cell_tables))
         return c
 
 
@@ -149,6 +151,11 @@ class NdtScanMatcherBuilder:
 
     def profile_phases(self, v: bool):
         return self._set(profile_phases=v)
+
+    def cell_tables(self, v: bool):
+        """Per-cell neighbour/slot tables for the lookups (identical results;
+        hash + bitmap only when off)."""
+        return self._set(cell_tables=v)
 
     def build(self) -> "NdtScanMatcher":
         return NdtScanMatcher.with_config(self.config)

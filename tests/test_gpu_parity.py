@@ -123,6 +123,34 @@ def test_lookup_parity(gpu_small, small, oracle_grid, mode):
     assert _coord_sets(g["coords"], cnt_g, idx_g) == _coord_sets(ex["coords"], cnt_o, idx_o)
 
 
+@pytest.mark.gpu
+def test_cell_tables_match_hash_lookups(gpu_small, small):
+    """The per-cell tables and the hash + bitmap path visit the same voxels in
+    the same order: lookups agree exactly; derivatives and alignments agree to
+    rounding (the two inlined paths may contract multiply-adds differently)."""
+    _print_case("cell tables vs hash")
+    hashed = _matcher(small.map_points, cell_tables=False)
+    rng = np.random.default_rng(11)
+    R = npref.rot_xyz(*small.guess[3:])
+    q = (small.scan.astype(np.float64) @ R.T + small.guess[:3]).astype(np.float32)
+    q = np.concatenate([q, (q[:500] + rng.normal(0, 1.0, (500, 3))).astype(np.float32),
+                        rng.uniform(-500, 500, (200, 3)).astype(np.float32)])
+    for mode in (0, 1, 2, 3):
+        c1, i1 = gpu_small.lookup(q, mode, max_k=40)
+        c2, i2 = hashed.lookup(q, mode, max_k=40)
+        assert np.array_equal(c1, c2) and np.array_equal(i1, i2), mode
+    s1, g1, H1, n1 = gpu_small.derivatives(small.scan, small.guess, True)
+    s2, g2, H2, n2 = hashed.derivatives(small.scan, small.guess, True)
+    assert n1 == n2 and s1 == pytest.approx(s2, rel=1e-12)
+    np.testing.assert_allclose(g1, g2, rtol=0, atol=1e-12 * np.abs(g2).max())
+    np.testing.assert_allclose(H1, H2, rtol=0, atol=1e-12 * np.abs(H2).max())
+    r1 = gpu_small.align(small.scan, small.guess)
+    r2 = hashed.align(small.scan, small.guess)
+    assert r1.status == r2.status and r1.iterations == r2.iterations
+    np.testing.assert_allclose(r1.pose_vector, r2.pose_vector, rtol=0, atol=1e-9)
+    assert r1.nvtl == pytest.approx(r2.nvtl, rel=1e-12)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_derivatives_parity(gpu_small, small, oracle_grid, oracle_mod, mode):
     _print_case(f"derivatives mode {mode}")
