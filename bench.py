@@ -7,7 +7,14 @@ scan already resident in HBM.  The map is built once on the GPU before timing
 
 Options: --gpus N (only 1 is supported: every run here uses one GPU),
 --steps K, --warmup W, --no-cpu (skip the CPU baseline), --particles P
-(also time the batched initial-pose search with P particles).
+(also time the batched initial-pose search with P particles),
+--calibrate-copy MIB (run one device copy of MIB MiB before the timed
+region: a kernel of known traffic for scaling rocprofv3 FETCH_SIZE and
+WRITE_SIZE counters).
+
+roofline.traffic is the HBM bytes per Newton launch measured with rocprofv3
+counters; it is read from profiles/newton_kernel_traffic.md (null when that
+file has no figure).
 """
 import argparse
 import json
@@ -39,6 +46,19 @@ def build_bytes(n, voxels, key_bits):
     return 12 * n + (12 * n + 4 * n) + (4 * n + 8 * n) + (passes - 1) * 16 * n + 8 * n + 20 * n + 64 * voxels
 
 
+def measured_traffic():
+    """HBM bytes per Newton launch from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "newton_kernel_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(wl, budget_s=12.0):
     from oracle import oracle as O
 
@@ -66,6 +86,8 @@ def main():
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--phases", action="store_true", help="also report the Newton kernel's phase profile")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -95,6 +117,13 @@ def main():
     for _ in range(args.warmup):
         r = m.align_device(ptr, n, wl.guess)
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy {args.calibrate_copy} MiB", file=sys.stderr, flush=True)
+        del src, dst
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -146,7 +175,7 @@ def main():
                    "map_points": int(wl.map_points.shape[0]), "scan_points": int(n), "voxels": int(voxels),
                    "iterations": int(r.iterations), "seed": args.seed},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "newton_kernel", "kernel_ms": k_ms, "flop_per_launch": flops},
         "map_build_ms": build_ms,
         "map_build_gbs": build_bytes(wl.map_points.shape[0], voxels, 22) / (build_ms * 1e-3) / 1e9,

@@ -153,9 +153,8 @@ int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, co
   NDT_HIP(hipMemcpyAsync(h->jobs.ptr, jobs.data(), m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
           "copy jobs");
   const ndt::NewtonConfig cfg = newton_config(h, W);
-  NDT_HIP(hipEventRecord(h->ev0, h->stream), "event");
   NDT_HIP(ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr), (int)m,
-                             reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream, W > 1),
+                             reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream, W > 1, h->ev0),
           "newton launch");
   NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
   NDT_HIP(hipMemcpyAsync(h->host_results, h->results.ptr, m * sizeof(ndt_result), hipMemcpyDeviceToHost, h->stream),

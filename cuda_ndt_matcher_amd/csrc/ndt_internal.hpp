@@ -95,10 +95,11 @@ static_assert(sizeof(ndt_result) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8, "ndt_result
 
 // Launches the persistent Newton kernel over `slots` jobs (device array) and
 // writes ndt_result per slot (device array).  ws must hold
-// newton_workspace_bytes(slots, wg_per_slot).
+// newton_workspace_bytes(slots, wg_per_slot).  ev_start (optional) is
+// recorded after the counter reset, just before the Newton kernel.
 size_t newton_workspace_bytes(int slots, int wg_per_slot);
 hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
-                         ndt_result* d_out, void* ws, hipStream_t st, bool cooperative);
+                         ndt_result* d_out, void* ws, hipStream_t st, bool cooperative, hipEvent_t ev_start);
 int newton_max_coresident_workgroups();
 
 // One derivative pass (score, gradient, Hessian, count) at a pose.

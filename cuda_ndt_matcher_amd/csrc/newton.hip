@@ -931,13 +931,17 @@ int newton_max_coresident_workgroups() {
 }
 
 hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
-                         ndt_result* d_out, void* ws, hipStream_t st, bool cooperative) {
+                         ndt_result* d_out, void* ws, hipStream_t st, bool cooperative, hipEvent_t ev_start) {
   const int W = cfg.wg_per_slot;
   double* partials = reinterpret_cast<double*>(ws);
   uint32_t* counters =
       reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
   const int ncounters = slots * counter_lines(W) * kCounterStride;
   hipLaunchKernelGGL(zero_u32, dim3(std::min(1024, (ncounters + 255) / 256)), dim3(256), 0, st, counters, ncounters);
+  if (ev_start) {
+    const hipError_t e = hipEventRecord(ev_start, st);
+    if (e != hipSuccess) return e;
+  }
   const dim3 grid((unsigned)(slots * W)), block(kBlock);
   if (cooperative) {
     GridView gg = g;

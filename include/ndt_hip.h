@@ -129,14 +129,14 @@ typedef struct ndt_timing {
 typedef struct ndt_phase_profile {
   float derivative_us;  /* per-point derivative passes (this workgroup's share) + block reduction */
   float exchange_us;    /* partial exchange and slot barrier (0 with one workgroup per slot) */
-  float solve_us;       /* 6x6 Newton step, convergence test and pose update (thread 0) */
+  float solve_us;       /* 6x6 Newton step (waves 0 and 1), convergence test and pose update */
   float line_search_us; /* More-Thuente evaluations (derivative passes without Hessian) */
   float nvtl_us;        /* final NVTL pass */
   float total_us;
   uint32_t iterations;
   uint32_t line_search_evaluations;
   float slowest_derivative_us; /* derivative passes of the slowest workgroup of the launch */
-  float solve_setup_us;        /* part of solve_us before the 6x6 step (thread 0 bookkeeping) */
+  float solve_setup_us;        /* part of solve_us before the 6x6 step (sums, best pose, regularisation) */
   float solve_step_us;         /* part of solve_us in the 6x6 step itself */
   uint32_t eigen_solves;       /* steps that needed the eigendecomposition (Hessian not negative definite) */
   uint32_t jacobi_sweeps;      /* Jacobi sweeps over those steps */

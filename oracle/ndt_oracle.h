@@ -9,6 +9,31 @@
  *
  * Every function cites the reference file:line it follows (paths relative to
  * the reference repository root).
+ *
+ * How each function is pinned (tests/test_oracle.py, tests/test_abi.py):
+ *   oracle_gauss              closed form of GaussianParams::new
+ *   oracle_point_keys         the reference's key cases and brute force at cell edges
+ *   oracle_grid_build/export  numpy covariance, the <6-point drop rule, the
+ *                             reference's cluster and planar cases, the f32 ratio
+ *                             of the eigen threshold
+ *   oracle_sym_eigen3/6       LAPACK (numpy.linalg.eigh)
+ *   oracle_search             brute force over all voxels; direct-mode cell sets
+ *   oracle_derivatives        score -d1 at the mean, central finite differences of
+ *                             score and gradient (both metrics)
+ *   oracle_newton_step        the reference's negative-definite cases, numpy solve
+ *   oracle_align              the reference's alignment tests (half-cubic cloud),
+ *                             best-pose rule at max iterations, line search
+ *   oracle_nvtl/_tp/per_point, oracle_nvtl_cell27   brute force
+ *   oracle_more_thuente       the reference's cases and the strong Wolfe conditions
+ *   oracle_regularization     finite differences
+ *   oracle_oscillation        the reference's cases
+ *   oracle_laplace_xy         closed-form 2x2 inverse
+ *   oracle_pose_to_matrix/matrix_to_pose   round trip and rotation-order cases
+ * Parity unpinned (the reference is Rust and cannot be built here, so these
+ * third-party results are restated, not compared bit for bit): the rounding
+ * of nalgebra 0.33's symmetric_eigen, SVD solve and Euler conversions, and
+ * kiddo 4.2's radius search at the exact boundary |q - mean|^2 == r^2 (taken
+ * here as excluded) and its order among equal distances.
  */
 #ifndef NDT_ORACLE_H
 #define NDT_ORACLE_H
