@@ -13,7 +13,7 @@ constexpr int kWave = 64;
 constexpr uint64_t kEmptyKey = ~0ull;
 constexpr int32_t kCoordBias = 1 << 20;  // voxel coordinates must lie in [-2^20, 2^20)
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
-constexpr uint64_t kCellTablesMaxCells = 1ull << 28;
+constexpr uint64_t kCellTablesMaxCells = 1ull << 26;  // 20 bytes per cell: 1.25 GiB at most
 
 // ---------------------------------------------------------------------------
 // Voxel grid as the kernels see it.  Voxel slots are in Morton-key order; a
@@ -28,11 +28,13 @@ constexpr uint64_t kCellTablesMaxCells = 1ull << 28;
 //    side (one bit per cell) when that box has fewer than 2^32 cells; it
 //    answers "is this cell occupied" for the 3x3x3 block of a query with nine
 //    independent word loads, so the hash is probed only for occupied cells;
-//  - cell tables over the same padded box when it has at most 2^28 cells:
+//  - cell tables over the same padded box when it has at most 2^26 cells:
 //    cell_nbr holds, per cell, the 27-bit occupancy of its 3x3x3 block and
-//    cell_slot the voxel slot of the cell (or kNoSlot).  A query then costs
-//    one load for its block and one per occupied candidate cell, with no
-//    probing.
+//    cell_ent, per occupied cell, the voxel mean and slot in 16 bytes (cells
+//    that cell_nbr marks empty are never read, so they are left unwritten).
+//    A query costs one load for its block, one 16-byte load per occupied
+//    candidate cell (enough for the radius test) and the record loads of
+//    the accepted ones, with no probing.
 struct __align__(32) HashEntry {
   unsigned long long key;
   uint32_t slot;
@@ -53,7 +55,7 @@ struct GridView {
   uint32_t table_mask;      // capacity - 1
   const uint32_t* bitmap;   // null when the padded box is too large
   const uint32_t* cell_nbr;   // null when the cell tables are not built
-  const uint32_t* cell_slot;
+  const float4* cell_ent;     // per occupied cell {mx, my, mz, slot bits}
   int32_t borg[3];          // padded box origin (cmin - 2)
   uint32_t bdim[3];         // padded box extent (cmax - cmin + 5)
   int32_t cmin[3];          // bounding box of the occupied voxel coordinates
@@ -178,34 +180,42 @@ __device__ inline uint32_t box_index(const GridView& g, int32_t x, int32_t y, in
 }
 
 // Cell-table visit of the candidate cells around the cell with box index
-// `lin`: four candidates per batch, so their slot loads and then their mean
-// loads are in flight together.  Candidates are visited in increasing cell
-// order, as in probe_cells.
+// `lin`, eight at a time: their 16-byte entries (mean, slot) are loaded
+// together, then the records of the accepted ones.  The gather rate of each
+// CU bounds these lookups, so the rejected candidates cost one load each.
+// Candidates are visited in increasing cell order, as in probe_cells.
 template <class A, class F>
 __device__ inline void probe_dense(const GridView& g, uint32_t lin, uint32_t cand, A&& accept, F&& visit) {
   const int32_t sy = (int32_t)g.bdim[0], sz = (int32_t)(g.bdim[0] * g.bdim[1]);
 #pragma unroll 1
   while (cand) {
-    uint32_t sl[4];
+    float4 ent[8];
+    bool ok[8];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      sl[b] = kNoSlot;
+    for (int b = 0; b < 8; ++b) {
+      ok[b] = false;
       if (cand) {
         const int c = __ffs(cand) - 1;
         cand &= cand - 1;
         const int32_t off = (c / 9 - 1) + (c / 3 % 3 - 1) * sy + (c % 3 - 1) * sz;
-        sl[b] = g.cell_slot[(uint32_t)((int32_t)lin + off)];
+        ent[b] = g.cell_ent[(uint32_t)((int32_t)lin + off)];
+        ok[b] = true;
       }
     }
-    // Whole 48-byte records, so the visit does not wait for a third
-    // dependent round trip (unused parts are dropped by the compiler).
-    VoxelRecord r[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (sl[b] != kNoSlot) r[b] = VoxelRecord{g.vox[3 * sl[b]], g.vox[3 * sl[b] + 1], g.vox[3 * sl[b] + 2]};
+    for (int b = 0; b < 8; ++b) ok[b] = ok[b] && accept(ent[b].x, ent[b].y, ent[b].z);
+    float4 m0[8], m1[8], m2[8];
 #pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (sl[b] != kNoSlot && accept(r[b].m0.x, r[b].m0.y, r[b].m0.z)) visit((int32_t)sl[b], r[b]);
+    for (int b = 0; b < 8; ++b)
+      if (ok[b]) {
+        const uint32_t sl = __float_as_uint(ent[b].w);
+        m0[b] = g.vox[3 * sl];
+        m1[b] = g.vox[3 * sl + 1];
+        m2[b] = g.vox[3 * sl + 2];
+      }
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if (ok[b]) visit((int32_t)__float_as_uint(ent[b].w), VoxelRecord{m0[b], m1[b], m2[b]});
   }
 }
 
@@ -219,7 +229,7 @@ __device__ inline void probe_block(const GridView& g, int32_t kx, int32_t ky, in
   if (kx < g.cmin[0] - 1 || kx > g.cmax[0] + 1 || ky < g.cmin[1] - 1 || ky > g.cmax[1] + 1 ||
       kz < g.cmin[2] - 1 || kz > g.cmax[2] + 1)
     return;
-  if (g.cell_slot) {
+  if (g.cell_ent) {
     const uint32_t lin = box_index(g, kx, ky, kz);
     probe_dense(g, lin, mask & g.cell_nbr[lin], accept, visit);
     return;
@@ -484,12 +494,14 @@ __device__ inline double block_sums(double (&v)[N], int count, double* lds) {
 // registers).  Each thread stores its values as column t of a value-major
 // table; then 8 threads per value sum 32 table entries each (stride 8) and
 // combine with three shuffles.  out[0..count) receives the totals (LDS,
-// synchronised on return).  red must hold count * (BLOCK + 8) doubles.
+// synchronised on return).  red must hold count * (BLOCK + 8) doubles; it may
+// alias LDS that other threads were still reading, hence the barrier first.
 template <int BLOCK>
 __device__ inline void block_sums_lds(const double* v, int count, double* red, double* out) {
   static_assert(BLOCK == 256, "8 threads x 32 entries per value");
   constexpr int kRow = BLOCK + 8;  // padding spreads the value rows over the banks
   const int t = threadIdx.x;
+  __syncthreads();
   for (int i = 0; i < count; ++i) red[i * kRow + t] = v[i];
   __syncthreads();
   const int val = t >> 3, part = t & 7;
@@ -737,7 +749,11 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
       }
       off = readlane_f64(off, 0);
       dg = readlane_f64(dg, 0);
-      if (off == 0.0 || off <= 1e-34 * dg) break;
+      // Stop once the off-diagonal norm is 1e-15 of the diagonal's: the
+      // eigenvalue error is then of order 1e-30 relative and the eigenvector
+      // error of order 1e-15 times the condition of the gaps, below what the
+      // step needs; one more sweep would only polish rounding.
+      if (off == 0.0 || off <= 1e-30 * dg) break;
       if (__hip_atomic_load(&W.fast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
         fast_exit = true;
         break;

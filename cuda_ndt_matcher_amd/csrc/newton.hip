@@ -201,11 +201,23 @@ __device__ inline void point_jacobian(const PoseTables& T, double px, double py,
 //   g_j = a . J_j,  H_ij = J_i^T M J_j + a . d2T/dp_i dp_j   (i >= j).
 // The point-to-plane metric (cpu.rs:246-329) has the same shape with
 // a = sum e d n and M = sum e (1 - d2 d^2) n n^T.
+//
+// The neighbours of a point are first staged (record and slot) in this
+// thread's column of an LDS table and then visited in one loop, so a wave
+// runs the pair computation max-over-lanes(neighbours) times rather than
+// once per candidate position that any lane accepted.
+constexpr int kStageK = 8;  // staged neighbours per thread before a flush
+struct Stage {
+  float4* rec;   // [kStageK][3][kBlock]
+  int32_t* slot; // [kStageK][kBlock]
+};
+
 template <bool kHess>
 __device__ inline void derivative_points(const GridView& g, const NewtonConfig& cfg, const PoseTables& T,
                                          const float* __restrict__ pts, uint32_t n, uint32_t first,
-                                         uint32_t stride, double* acc, uint32_t& corr) {
+                                         uint32_t stride, double* acc, uint32_t& corr, const Stage& st) {
   const double d1 = cfg.d1, d2 = cfg.d2;
+  const uint32_t tid = threadIdx.x;
   for (uint32_t i = first; i < n; i += stride) {
     // Keeps the pose tables in LDS until they are used: a thread usually has
     // one point, and hoisting the 70 table entries out of this loop would
@@ -221,8 +233,7 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
 #pragma unroll
     for (int k = 0; k < 9; ++k) M[k] = 0.0;
     uint32_t np = 0;
-    for_each_neighbor(g, cfg.search_mode, (float)t[0], (float)t[1], (float)t[2],
-                      [&](int32_t s, const VoxelRecord& rec) {
+    auto pair = [&](int32_t s, VoxelRecord rec) {
       const double x0 = t[0] - (double)rec.m0.x, x1 = t[1] - (double)rec.m0.y, x2 = t[2] - (double)rec.m0.z;
       ++corr;
       ++np;
@@ -270,7 +281,29 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
             for (int c = 0; c < 3; ++c) M[3 * r + c] += e * C[3 * c + r] + ed * (xc[r] * xc[c]);
         }
       }
-    });
+    };
+    // Neighbours kStageK at a time, in enumeration order; a point with more
+    // (rare: at most 27 exist) enumerates again for the next ones.
+    for (uint32_t skip = 0;; skip += kStageK) {
+      uint32_t seen = 0, staged = 0;
+      for_each_neighbor(g, cfg.search_mode, (float)t[0], (float)t[1], (float)t[2],
+                        [&](int32_t s, VoxelRecord rec) {
+        if (seen >= skip && staged < kStageK) {
+          st.rec[(3 * staged) * kBlock + tid] = rec.m0;
+          st.rec[(3 * staged + 1) * kBlock + tid] = rec.m1;
+          st.rec[(3 * staged + 2) * kBlock + tid] = rec.m2;
+          st.slot[staged * kBlock + tid] = s;
+          ++staged;
+        }
+        ++seen;
+      });
+      for (uint32_t v = 0; v < staged; ++v) {
+        const VoxelRecord r{st.rec[(3 * v) * kBlock + tid], st.rec[(3 * v + 1) * kBlock + tid],
+                            st.rec[(3 * v + 2) * kBlock + tid]};
+        pair(st.slot[v * kBlock + tid], r);
+      }
+      if (seen <= skip + kStageK) break;
+    }
     if (np == 0) continue;
     double Jr[3][3], ht[15];
     point_jacobian(T, px, py, pz, Jr, ht, kHess);
@@ -337,7 +370,7 @@ __device__ inline void nvtl_points(const GridView& g, double d1, double d2, cons
     iso_transform(q, tr, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], x);
     double best = -HUGE_VAL;
     bool any = false;
-    for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, const VoxelRecord& r) {
+    for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, VoxelRecord r) {
       const double sc = point_score(x, r.m0, r.m1, r.m2, d1, d2);
       best = fmax(best, sc);
       any = true;
@@ -518,8 +551,16 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   __shared__ SlotState S;
   __shared__ SolveScratch s_solve;
   __shared__ double s_scratch[kReduceLds];
-  __shared__ double s_red[(kAccVals + 1) * (kBlock + 8)];
+  // The derivative passes stage neighbours here; the workgroup reduction
+  // reuses the same space after them.
+  __shared__ union {
+    float4 rec[kStageK * 3 * kBlock];
+    double red[(kAccVals + 1) * (kBlock + 8)];
+  } s_big;
+  __shared__ int32_t s_stage_slot[kStageK * kBlock];
   __shared__ double s_tot[kAccSlots];
+  double* const s_red = s_big.red;
+  const Stage stage{s_big.rec, s_stage_slot};
   __shared__ double s_q[4];
 
   const int W = cfg.wg_per_slot;
@@ -587,7 +628,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 #pragma unroll
     for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
     uint32_t corr = 0;
-    derivative_points<true>(g, cfg, s_tab, job.pts, last, first, stride, acc, corr);
+    derivative_points<true>(g, cfg, s_tab, job.pts, last, first, stride, acc, corr, stage);
     acc[kAccVals] = (double)corr;
     slot_reduce(acc, kAccVals + 1, s_scratch, s_red, s_tot, sy, rec_all ? &t_blk : nullptr);
     if (rec_all) own_deriv += t_blk - t_own;
@@ -700,7 +741,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 #pragma unroll
         for (int i = 0; i < 8; ++i) la[i] = 0.0;
         uint32_t lc = 0;
-        derivative_points<false>(g, cfg, s_tab, job.pts, last, first, stride, la, lc);
+        derivative_points<false>(g, cfg, s_tab, job.pts, last, first, stride, la, lc, stage);
         la[7] = (double)lc;
         slot_reduce(la, 8, s_scratch, s_red, s_tot, sy);
         if (threadIdx.x == 0) {
@@ -806,15 +847,18 @@ __global__ __launch_bounds__(kBlock) void derivative_kernel(GridView g, NewtonCo
                                                             uint32_t n, PoseTables T, int hess,
                                                             double* __restrict__ partials) {
   __shared__ double s_scratch[(kBlock / kWave) * kAccSlots];
+  __shared__ float4 s_stage_rec[kStageK * 3 * kBlock];
+  __shared__ int32_t s_stage_slot[kStageK * kBlock];
+  const Stage stage{s_stage_rec, s_stage_slot};
   double acc[kAccSlots];
 #pragma unroll
   for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
   uint32_t corr = 0;
   const uint32_t first = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
   if (hess)
-    derivative_points<true>(g, cfg, T, pts, n, first, stride, acc, corr);
+    derivative_points<true>(g, cfg, T, pts, n, first, stride, acc, corr, stage);
   else
-    derivative_points<false>(g, cfg, T, pts, n, first, stride, acc, corr);
+    derivative_points<false>(g, cfg, T, pts, n, first, stride, acc, corr, stage);
   acc[kAccVals] = (double)corr;
   const double tot = block_sums<kBlock, kAccSlots>(acc, kAccVals + 1, s_scratch);
   if (threadIdx.x < kAccVals + 1) partials[blockIdx.x * kPartialStride + threadIdx.x] = tot;
@@ -854,7 +898,7 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     if (kind == kScoreNvtl || kind == kScoreNvtlCell27) {
       double best = -HUGE_VAL;
       bool any = false;
-      auto vis = [&](int32_t, const VoxelRecord& r) {
+      auto vis = [&](int32_t, VoxelRecord r) {
         best = fmax(best, point_score(x, r.m0, r.m1, r.m2, d1, d2));
         any = true;
       };
@@ -869,7 +913,7 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     } else {
       double ps = 0.0;
       double pc = 0.0;
-      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, const VoxelRecord& r) {
+      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, VoxelRecord r) {
         ps += point_score(x, r.m0, r.m1, r.m2, d1, d2);
         pc += 1.0;
       });
@@ -901,7 +945,7 @@ __global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, co
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   uint32_t c = 0;
-  auto vis = [&](int32_t s, const VoxelRecord&) {
+  auto vis = [&](int32_t s, VoxelRecord) {
     if (c < max_k) idx[(size_t)i * max_k + c] = s;
     ++c;
   };

@@ -460,13 +460,16 @@ __global__ __launch_bounds__(kBuildBlock) void bitmap_set(const int32_t* __restr
   atomicOr(&bitmap[lin >> 5], 1u << (lin & 31));
 }
 
-// Cell tables: slot of every valid voxel at its cell ...
-__global__ __launch_bounds__(kBuildBlock) void cell_slot_set(const int32_t* __restrict__ coords,
-                                                             const uint32_t* __restrict__ valid, uint32_t segments,
-                                                             GridView v, uint32_t* cell_slot) {
+// Cell tables: mean and slot of every valid voxel at its cell ...
+__global__ __launch_bounds__(kBuildBlock) void cell_ent_set(const int32_t* __restrict__ coords,
+                                                            const uint32_t* __restrict__ valid,
+                                                            const float4* __restrict__ vox, uint32_t segments,
+                                                            GridView v, float4* cell_ent) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= segments || !valid[s]) return;
-  cell_slot[box_index(v, coords[3 * s], coords[3 * s + 1], coords[3 * s + 2])] = s;
+  const float4 m = vox[3 * s];
+  cell_ent[box_index(v, coords[3 * s], coords[3 * s + 1], coords[3 * s + 2])] =
+      make_float4(m.x, m.y, m.z, __uint_as_float(s));
 }
 
 // ... and the 27-bit block occupancy of every cell whose block lies inside
@@ -689,7 +692,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   v.res = res;
   v.bitmap = nullptr;
   v.cell_nbr = nullptr;
-  v.cell_slot = nullptr;
+  v.cell_ent = nullptr;
   if (G.num_valid > 0) {
     // Occupancy bitmap over the box padded by two cells, when it has < 2^32 cells.
     uint64_t cells = 1;
@@ -713,17 +716,15 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
       if (cell_tables && cells <= kCellTablesMaxCells) {
         const uint32_t nc = (uint32_t)cells;
         if ((e = G.cell_nbr.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
-        if ((e = G.cell_slot.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
-        hipLaunchKernelGGL(fill_u32, dim3(grid_for(nc, 256, 4096)), dim3(256), 0, st,
-                           reinterpret_cast<uint32_t*>(G.cell_slot.ptr), (size_t)nc, kNoSlot);
-        hipLaunchKernelGGL(cell_slot_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+        if ((e = G.cell_ent.reserve((size_t)nc * 16)) != hipSuccess) return fail(e, "alloc cell tables");
+        hipLaunchKernelGGL(cell_ent_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
                            reinterpret_cast<const int32_t*>(G.coord.ptr),
-                           reinterpret_cast<const uint32_t*>(G.valid.ptr), segs, v,
-                           reinterpret_cast<uint32_t*>(G.cell_slot.ptr));
+                           reinterpret_cast<const uint32_t*>(G.valid.ptr), reinterpret_cast<const float4*>(G.vox.ptr),
+                           segs, v, reinterpret_cast<float4*>(G.cell_ent.ptr));
         hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for(nc, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st, v, nc,
                            reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
         v.cell_nbr = reinterpret_cast<const uint32_t*>(G.cell_nbr.ptr);
-        v.cell_slot = reinterpret_cast<const uint32_t*>(G.cell_slot.ptr);
+        v.cell_ent = reinterpret_cast<const float4*>(G.cell_ent.ptr);
       }
     }
   }

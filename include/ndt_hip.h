@@ -84,9 +84,9 @@ typedef struct ndt_config {
   int32_t precision;                  /* ndt_precision, default NDT_PRECISION_FP64 */
   int32_t workgroups_per_alignment;   /* 0 = automatic; otherwise workgroups cooperating on one alignment */
   int32_t profile_phases;             /* 1 = time the Newton kernel's phases (see ndt_hip_get_phase_profile) */
-  int32_t cell_tables;                /* 1 = per-cell neighbour/slot tables when the map's padded voxel box has
-                                         at most 2^28 cells (8 bytes per cell); 0 = hash + bitmap lookups only.
-                                         Both give identical results. */
+  int32_t cell_tables;                /* 1 = per-cell neighbour tables when the map's padded voxel box has
+                                         at most 2^26 cells (20 bytes per cell); 0 = hash + bitmap lookups only.
+                                         Both give the same neighbours in the same order. */
 } ndt_config;
 
 /* Convergence status (optimization/types.rs:66-82). */
