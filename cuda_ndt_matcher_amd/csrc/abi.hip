@@ -272,7 +272,7 @@ int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
   *out = nullptr;
   std::string why;
   if (!valid_config(cfg, &why)) return NDT_ERR_INVALID_ARGUMENT;
-  if (cfg->precision != NDT_PRECISION_FP64) return NDT_ERR_UNSUPPORTED;
+  if (cfg->precision != NDT_PRECISION_FP64 && cfg->precision != NDT_PRECISION_FP32) return NDT_ERR_UNSUPPORTED;
   auto* h = new ndt_handle();
   h->cfg = *cfg;
   gauss_params((double)cfg->resolution, cfg->outlier_ratio, &h->d1, &h->d2);

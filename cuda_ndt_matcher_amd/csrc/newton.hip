@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "ndt_device.hpp"
 #include "ndt_internal.hpp"
@@ -171,23 +172,62 @@ __device__ void mt_feed(MTState& s, double score_t, double deriv_t) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-point work.
-__device__ inline void point_jacobian(const PoseTables& T, double px, double py, double pz, double Jr[3][3],
-                                      double ht[15], bool hess) {
-  double gt[8];
+// Per-point work.  R is the arithmetic type of the per-point terms: double
+// for the FP64 form, float for the FP32 form.
+template <class R>
+__device__ inline void point_jacobian(const PoseTables& T, R px, R py, R pz, R Jr[3][3], R ht[15], bool hess) {
+  R gt[8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) gt[r] = T.j[r][0] * px + T.j[r][1] * py + T.j[r][2] * pz;
-  Jr[0][0] = 0.0;   Jr[1][0] = gt[0]; Jr[2][0] = gt[1];
+  for (int r = 0; r < 8; ++r) gt[r] = (R)T.j[r][0] * px + (R)T.j[r][1] * py + (R)T.j[r][2] * pz;
+  Jr[0][0] = 0;     Jr[1][0] = gt[0]; Jr[2][0] = gt[1];
   Jr[0][1] = gt[2]; Jr[1][1] = gt[3]; Jr[2][1] = gt[4];
   Jr[0][2] = gt[5]; Jr[1][2] = gt[6]; Jr[2][2] = gt[7];
   if (hess) {
 #pragma unroll
-    for (int r = 0; r < 15; ++r) ht[r] = T.h[r][0] * px + T.h[r][1] * py + T.h[r][2] * pz;
+    for (int r = 0; r < 15; ++r) ht[r] = (R)T.h[r][0] * px + (R)T.h[r][1] * py + (R)T.h[r][2] * pz;
   } else {
 #pragma unroll
-    for (int r = 0; r < 15; ++r) ht[r] = 0.0;
+    for (int r = 0; r < 15; ++r) ht[r] = 0;
   }
 }
+
+// Per-thread accumulation of the 28 sums.  FP64: straight into acc.  FP32:
+// compensated (two-sum) float pairs, added to acc in double at the end of
+// the pass; a thread can hold many points (batched launches), and the
+// compensation keeps their float sum exact to about 2^-48 relative.
+template <class R>
+struct PointSums;
+
+template <>
+struct PointSums<double> {
+  double* acc;
+  __device__ explicit PointSums(double* a) : acc(a) {}
+  __device__ void add(int i, double x) { acc[i] += x; }
+  __device__ void finish() {}
+};
+
+template <>
+struct PointSums<float> {
+  double* acc;
+  float hi[kAccVals], lo[kAccVals];
+  __device__ explicit PointSums(double* a) : acc(a) {
+#pragma unroll
+    for (int i = 0; i < kAccVals; ++i) hi[i] = lo[i] = 0.0f;
+  }
+  __device__ void add(int i, float x) {
+    const float s = hi[i] + x;
+    const float bb = s - hi[i];
+    lo[i] += (hi[i] - (s - bb)) + (x - bb);
+    hi[i] = s;
+  }
+  __device__ void finish() {
+#pragma unroll
+    for (int i = 0; i < kAccVals; ++i) acc[i] += (double)hi[i] + (double)lo[i];
+  }
+};
+
+__device__ inline double exp_r(double x) { return exp(x); }
+__device__ inline float exp_r(float x) { return expf(x); }
 
 // Accumulates the derivative contributions of points first, first + stride,
 // ... below n.
@@ -212,12 +252,13 @@ struct Stage {
   int32_t* slot; // [kStageK][kBlock]
 };
 
-template <bool kHess>
+template <bool kHess, class R = double>
 __device__ inline void derivative_points(const GridView& g, const NewtonConfig& cfg, const PoseTables& T,
                                          const float* __restrict__ pts, uint32_t n, uint32_t first,
                                          uint32_t stride, double* acc, uint32_t& corr, const Stage& st) {
-  const double d1 = cfg.d1, d2 = cfg.d2;
+  const R d1 = (R)cfg.d1, d2 = (R)cfg.d2;
   const uint32_t tid = threadIdx.x;
+  PointSums<R> sums(acc);
   for (uint32_t i = first; i < n; i += stride) {
     // Keeps the pose tables in LDS until they are used: a thread usually has
     // one point, and hoisting the 70 table entries out of this loop would
@@ -228,30 +269,31 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
     t[0] = T.R[0] * px + T.R[1] * py + T.R[2] * pz + T.t[0];
     t[1] = T.R[3] * px + T.R[4] * py + T.R[5] * pz + T.t[1];
     t[2] = T.R[6] * px + T.R[7] * py + T.R[8] * pz + T.t[2];
-    double sc = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    double M[9];
+    R sc = 0, a0 = 0, a1 = 0, a2 = 0;
+    R M[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) M[k] = 0.0;
+    for (int k = 0; k < 9; ++k) M[k] = 0;
     uint32_t np = 0;
     auto pair = [&](int32_t s, VoxelRecord rec) {
-      const double x0 = t[0] - (double)rec.m0.x, x1 = t[1] - (double)rec.m0.y, x2 = t[2] - (double)rec.m0.z;
+      // residual in double (the point is placed in double), then R
+      const R x0 = (R)(t[0] - (double)rec.m0.x), x1 = (R)(t[1] - (double)rec.m0.y), x2 = (R)(t[2] - (double)rec.m0.z);
       ++corr;
       ++np;
       if (cfg.metric == 1) {
-        const double n0 = g.axis[3 * s], n1 = g.axis[3 * s + 1], n2 = g.axis[3 * s + 2];
-        const double d = x0 * n0 + x1 * n1 + x2 * n2;
-        const double dsq = d * d;
-        const double e_raw = exp(-d2 * dsq * 0.5);
-        const double nz = d2 * e_raw;
-        if (!(nz >= 0.0 && nz <= 1.0) || !isfinite(nz)) return;  // counted, contributes zero
-        const double e = d1 * d2 * e_raw;
+        const R n0 = g.axis[3 * s], n1 = g.axis[3 * s + 1], n2 = g.axis[3 * s + 2];
+        const R d = x0 * n0 + x1 * n1 + x2 * n2;
+        const R dsq = d * d;
+        const R e_raw = exp_r(-d2 * dsq * (R)0.5);
+        const R nz = d2 * e_raw;
+        if (!(nz >= (R)0 && nz <= (R)1) || !isfinite(nz)) return;  // counted, contributes zero
+        const R e = d1 * d2 * e_raw;
         sc += -d1 * e_raw;
         a0 += e * d * n0;
         a1 += e * d * n1;
         a2 += e * d * n2;
         if (kHess) {
-          const double f = e * (-d2 * dsq + 1.0);
-          const double nn[3] = {n0, n1, n2};
+          const R f = e * (-d2 * dsq + (R)1);
+          const R nn[3] = {n0, n1, n2};
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -259,22 +301,22 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
         }
       } else {
         const float4 m0 = rec.m0, m1 = rec.m1, m2 = rec.m2;
-        const double C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
-        const double xc0 = x0 * C[0] + x1 * C[3] + x2 * C[6];
-        const double xc1 = x0 * C[1] + x1 * C[4] + x2 * C[7];
-        const double xc2 = x0 * C[2] + x1 * C[5] + x2 * C[8];
-        const double xcx = xc0 * x0 + xc1 * x1 + xc2 * x2;
-        const double e_raw = exp(-d2 * xcx * 0.5);
-        const double nz = d2 * e_raw;
-        if (!(nz >= 0.0 && nz <= 1.0) || !isfinite(nz)) return;  // counted, contributes zero
-        const double e = nz * d1;
+        const R C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
+        const R xc0 = x0 * C[0] + x1 * C[3] + x2 * C[6];
+        const R xc1 = x0 * C[1] + x1 * C[4] + x2 * C[7];
+        const R xc2 = x0 * C[2] + x1 * C[5] + x2 * C[8];
+        const R xcx = xc0 * x0 + xc1 * x1 + xc2 * x2;
+        const R e_raw = exp_r(-d2 * xcx * (R)0.5);
+        const R nz = d2 * e_raw;
+        if (!(nz >= (R)0 && nz <= (R)1) || !isfinite(nz)) return;  // counted, contributes zero
+        const R e = nz * d1;
         sc += -d1 * e_raw;
         a0 += e * xc0;
         a1 += e * xc1;
         a2 += e * xc2;
         if (kHess) {
-          const double xc[3] = {xc0, xc1, xc2};
-          const double ed = -d2 * e;
+          const R xc[3] = {xc0, xc1, xc2};
+          const R ed = -d2 * e;
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -305,49 +347,50 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
       if (seen <= skip + kStageK) break;
     }
     if (np == 0) continue;
-    double Jr[3][3], ht[15];
-    point_jacobian(T, px, py, pz, Jr, ht, kHess);
-    acc[0] += sc;
-    acc[1] += a0;
-    acc[2] += a1;
-    acc[3] += a2;
+    R Jr[3][3], ht[15];
+    point_jacobian<R>(T, (R)px, (R)py, (R)pz, Jr, ht, kHess);
+    sums.add(0, sc);
+    sums.add(1, a0);
+    sums.add(2, a1);
+    sums.add(3, a2);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) acc[4 + c] += a0 * Jr[0][c] + a1 * Jr[1][c] + a2 * Jr[2][c];
+    for (int c = 0; c < 3; ++c) sums.add(4 + c, a0 * Jr[0][c] + a1 * Jr[1][c] + a2 * Jr[2][c]);
     if (kHess) {
-      double MJ[3][3];  // M * Jr
+      R MJ[3][3];  // M * Jr
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c) MJ[r][c] = M[3 * r] * Jr[0][c] + M[3 * r + 1] * Jr[1][c] + M[3 * r + 2] * Jr[2][c];
       // translation block
-      acc[7 + tri(0, 0)] += M[0];
-      acc[7 + tri(1, 0)] += M[3];
-      acc[7 + tri(1, 1)] += M[4];
-      acc[7 + tri(2, 0)] += M[6];
-      acc[7 + tri(2, 1)] += M[7];
-      acc[7 + tri(2, 2)] += M[8];
+      sums.add(7 + tri(0, 0), M[0]);
+      sums.add(7 + tri(1, 0), M[3]);
+      sums.add(7 + tri(1, 1), M[4]);
+      sums.add(7 + tri(2, 0), M[6]);
+      sums.add(7 + tri(2, 1), M[7]);
+      sums.add(7 + tri(2, 2), M[8]);
       // rotation rows against translation columns: sum_k Jr[k][i] M[k][j]
 #pragma unroll
       for (int ri = 0; ri < 3; ++ri)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          acc[7 + tri(3 + ri, j)] += Jr[0][ri] * M[j] + Jr[1][ri] * M[3 + j] + Jr[2][ri] * M[6 + j];
+          sums.add(7 + tri(3 + ri, j), Jr[0][ri] * M[j] + Jr[1][ri] * M[3 + j] + Jr[2][ri] * M[6 + j]);
       // rotation block plus the point-Hessian terms a . d2T/dp_i dp_j
-      const double pa = a1 * ht[0] + a2 * ht[1];
-      const double pb = a1 * ht[2] + a2 * ht[3];
-      const double pc = a1 * ht[4] + a2 * ht[5];
-      const double pd = a0 * ht[6] + a1 * ht[7] + a2 * ht[8];
-      const double pe = a0 * ht[9] + a1 * ht[10] + a2 * ht[11];
-      const double pf = a0 * ht[12] + a1 * ht[13] + a2 * ht[14];
-      const double ph[3][3] = {{pa, pb, pc}, {pb, pd, pe}, {pc, pe, pf}};
+      const R pa = a1 * ht[0] + a2 * ht[1];
+      const R pb = a1 * ht[2] + a2 * ht[3];
+      const R pc = a1 * ht[4] + a2 * ht[5];
+      const R pd = a0 * ht[6] + a1 * ht[7] + a2 * ht[8];
+      const R pe = a0 * ht[9] + a1 * ht[10] + a2 * ht[11];
+      const R pf = a0 * ht[12] + a1 * ht[13] + a2 * ht[14];
+      const R ph[3][3] = {{pa, pb, pc}, {pb, pd, pe}, {pc, pe, pf}};
 #pragma unroll
       for (int ri = 0; ri < 3; ++ri)
 #pragma unroll
         for (int rj = 0; rj <= ri; ++rj)
-          acc[7 + tri(3 + ri, 3 + rj)] +=
-              Jr[0][ri] * MJ[0][rj] + Jr[1][ri] * MJ[1][rj] + Jr[2][ri] * MJ[2][rj] + ph[ri][rj];
+          sums.add(7 + tri(3 + ri, 3 + rj),
+                   Jr[0][ri] * MJ[0][rj] + Jr[1][ri] * MJ[1][rj] + Jr[2][ri] * MJ[2][rj] + ph[ri][rj]);
     }
   }
+  sums.finish();
 }
 
 __device__ inline double point_score(const double x[3], const float4& m0, const float4& m1, const float4& m2,
@@ -544,6 +587,8 @@ __device__ uint32_t oscillation_count(const double* h, int k) {
   return mx;
 }
 
+// kF32: per-point terms in FP32 (the precision variant); FP64 otherwise.
+template <bool kF32>
 __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig cfg, const SlotJob* __restrict__ jobs,
                                                         ndt_result* __restrict__ out, double* __restrict__ partials,
                                                         uint32_t* __restrict__ counters) {
@@ -628,7 +673,8 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 #pragma unroll
     for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
     uint32_t corr = 0;
-    derivative_points<true>(g, cfg, s_tab, job.pts, last, first, stride, acc, corr, stage);
+    derivative_points<true, typename std::conditional<kF32, float, double>::type>(g, cfg, s_tab, job.pts, last,
+                                                                                   first, stride, acc, corr, stage);
     acc[kAccVals] = (double)corr;
     slot_reduce(acc, kAccVals + 1, s_scratch, s_red, s_tot, sy, rec_all ? &t_blk : nullptr);
     if (rec_all) own_deriv += t_blk - t_own;
@@ -741,7 +787,8 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 #pragma unroll
         for (int i = 0; i < 8; ++i) la[i] = 0.0;
         uint32_t lc = 0;
-        derivative_points<false>(g, cfg, s_tab, job.pts, last, first, stride, la, lc, stage);
+        derivative_points<false, typename std::conditional<kF32, float, double>::type>(
+            g, cfg, s_tab, job.pts, last, first, stride, la, lc, stage);
         la[7] = (double)lc;
         slot_reduce(la, 8, s_scratch, s_red, s_tot, sy);
         if (threadIdx.x == 0) {
@@ -843,6 +890,7 @@ __global__ void zero_u32(uint32_t* p, int n) {
 
 // ---------------------------------------------------------------------------
 // Single derivative pass (parity entry point).
+template <bool kF32>
 __global__ __launch_bounds__(kBlock) void derivative_kernel(GridView g, NewtonConfig cfg, const float* __restrict__ pts,
                                                             uint32_t n, PoseTables T, int hess,
                                                             double* __restrict__ partials) {
@@ -855,10 +903,11 @@ __global__ __launch_bounds__(kBlock) void derivative_kernel(GridView g, NewtonCo
   for (int i = 0; i < kAccSlots; ++i) acc[i] = 0.0;
   uint32_t corr = 0;
   const uint32_t first = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+  using R = typename std::conditional<kF32, float, double>::type;
   if (hess)
-    derivative_points<true>(g, cfg, T, pts, n, first, stride, acc, corr, stage);
+    derivative_points<true, R>(g, cfg, T, pts, n, first, stride, acc, corr, stage);
   else
-    derivative_points<false>(g, cfg, T, pts, n, first, stride, acc, corr, stage);
+    derivative_points<false, R>(g, cfg, T, pts, n, first, stride, acc, corr, stage);
   acc[kAccVals] = (double)corr;
   const double tot = block_sums<kBlock, kAccSlots>(acc, kAccVals + 1, s_scratch);
   if (threadIdx.x < kAccVals + 1) partials[blockIdx.x * kPartialStride + threadIdx.x] = tot;
@@ -968,10 +1017,13 @@ int newton_max_coresident_workgroups() {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(newton_kernel), kBlock,
-                                                   0) != hipSuccess)
+  int per_cu32 = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(newton_kernel<false>),
+                                                   kBlock, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu32, reinterpret_cast<const void*>(newton_kernel<true>),
+                                                   kBlock, 0) != hipSuccess)
     return 0;
-  return cus * per_cu;
+  return cus * std::min(per_cu, per_cu32);
 }
 
 hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
@@ -993,9 +1045,14 @@ hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJ
     const SlotJob* jj = d_jobs;
     ndt_result* oo = d_out;
     void* args[] = {&gg, &cc, &jj, &oo, &partials, &counters};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(newton_kernel), grid, block, args, 0, st);
+    const void* fn = cfg.precision == NDT_PRECISION_FP32 ? reinterpret_cast<const void*>(newton_kernel<true>)
+                                                         : reinterpret_cast<const void*>(newton_kernel<false>);
+    return hipLaunchCooperativeKernel(fn, grid, block, args, 0, st);
   }
-  hipLaunchKernelGGL(newton_kernel, grid, block, 0, st, g, cfg, d_jobs, d_out, partials, counters);
+  if (cfg.precision == NDT_PRECISION_FP32)
+    hipLaunchKernelGGL(newton_kernel<true>, grid, block, 0, st, g, cfg, d_jobs, d_out, partials, counters);
+  else
+    hipLaunchKernelGGL(newton_kernel<false>, grid, block, 0, st, g, cfg, d_jobs, d_out, partials, counters);
   return hipGetLastError();
 }
 
@@ -1053,7 +1110,12 @@ hipError_t launch_derivatives(const GridView& g, const NewtonConfig& cfg, const 
     H[13][0] = -cx * sz - sx * s_y * cz; H[13][1] = -cx * cz + sx * s_y * sz; H[13][2] = 0.0;
     H[14][0] = -sx * sz + cx * s_y * cz; H[14][1] = -cx * s_y * sz - sx * cz; H[14][2] = 0.0;
   }
-  hipLaunchKernelGGL(derivative_kernel, dim3(blocks), dim3(kBlock), 0, st, g, cfg, d_pts, n, T, hess, d_partials);
+  if (cfg.precision == NDT_PRECISION_FP32)
+    hipLaunchKernelGGL(derivative_kernel<true>, dim3(blocks), dim3(kBlock), 0, st, g, cfg, d_pts, n, T, hess,
+                       d_partials);
+  else
+    hipLaunchKernelGGL(derivative_kernel<false>, dim3(blocks), dim3(kBlock), 0, st, g, cfg, d_pts, n, T, hess,
+                       d_partials);
   hipLaunchKernelGGL(reduce_partials, dim3(1, 1), dim3(64), 0, st, d_partials, blocks, kAccVals + 1, d_out);
   return hipGetLastError();
 }

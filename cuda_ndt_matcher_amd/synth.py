@@ -130,6 +130,26 @@ def perturb(pose: np.ndarray, seed: int, sigma_t: float, sigma_deg: float, rpy: 
 _SCENES = {}
 
 
+def sweep_scans(seed: int, sizes, leaf: float = 0.1):
+    """Scans of the given sizes over the small workload's map (the precision
+    sweep): one 128-beam scan from the small workload's sensor pose,
+    downsampled at `leaf`, then sub-sampled without replacement to each size
+    (seeded).  Returns (workload, [scan per size])."""
+    wl = make_workload("small", seed)
+    extent = WORKLOADS["small"][0]
+    scene = Scene(seed, extent, WORKLOADS["small"][1], WORKLOADS["small"][2])
+    raw = scene.lidar(wl.true_pose[0], wl.true_pose[1], wl.true_pose[2], wl.true_pose[5], 128, seed=seed + 1)
+    dense = voxel_downsample(raw, leaf)
+    r = rng(seed, 13)
+    out = []
+    for n in sizes:
+        if n > dense.shape[0]:
+            raise ValueError(f"only {dense.shape[0]} points available for a {n}-point scan")
+        idx = np.sort(r.choice(dense.shape[0], n, replace=False))
+        out.append(np.ascontiguousarray(dense[idx]))
+    return wl, out
+
+
 def make_workload(name: str, seed: int = 0, map_points: int | None = None) -> Workload:
     extent, nb, npl, nmap, beams, leaf, st, sd, rpy = WORKLOADS[name]
     n = nmap if map_points is None else map_points

@@ -211,6 +211,37 @@ def oracle_stable(og, scan, pose, params, tol=1e-5, noise=1e-12):
     return o, True
 
 
+def test_fp32_derivatives_close_to_fp64_oracle(small, oracle_grid, oracle_mod):
+    """Precision variant: per-point terms in FP32 with compensated sums.  The
+    neighbour sets are the FP64 ones (the lookups do not change); score,
+    gradient and Hessian agree with the FP64 oracle to FP32 rounding of the
+    per-pair terms (tolerance 2e-5 of the largest entry)."""
+    _print_case("fp32 derivatives")
+    gpu = _matcher(small.map_points, precision=1)
+    rng = np.random.default_rng(31)
+    p = oracle_mod.params()
+    for k in range(3):
+        pose = small.guess + np.concatenate([rng.normal(0, 0.2, 3), rng.normal(0, 0.02, 3)])
+        s_g, g_g, H_g, c_g = gpu.derivatives(small.scan, pose, True)
+        s_o, g_o, H_o, c_o = oracle_grid.derivatives(small.scan, pose, p, True)
+        assert c_g == c_o
+        assert s_g == pytest.approx(s_o, rel=2e-5)
+        np.testing.assert_allclose(g_g, g_o, rtol=0, atol=2e-5 * np.abs(g_o).max())
+        np.testing.assert_allclose(H_g, H_o, rtol=0, atol=2e-5 * np.abs(H_o).max())
+
+
+def test_fp32_alignment_matches_fp64_oracle(small, oracle_grid, oracle_mod):
+    """Precision variant on the small workload: same status and iteration
+    count as the FP64 oracle, pose within 1 mm / 0.01 deg."""
+    _print_case("fp32 align")
+    p = oracle_mod.params()
+    o, stable = oracle_stable(oracle_grid, small.scan, small.guess, p, noise=1e-6)
+    if not stable:
+        pytest.skip("start is not stable under FP32-size perturbations")
+    r = _matcher(small.map_points, precision=1).align(small.scan, small.guess)
+    _compare_align(r, o, small.scan.shape[0])
+
+
 @pytest.mark.parametrize("mode,ls", [(0, False), (1, False), (2, False), (3, False), (0, True)])
 def test_align_parity_small(small, oracle_grid, oracle_mod, mode, ls):
     _print_case(f"align small mode {mode} ls {ls}")
