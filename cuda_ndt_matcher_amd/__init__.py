@@ -7,7 +7,7 @@ crate's NdtScanMatcher API plus seeded synthetic workloads.
 from .isometry import Isometry3
 from .matcher import (METRIC_POINT_TO_DISTRIBUTION, METRIC_POINT_TO_PLANE, PRECISION_FP32, PRECISION_FP64,
                       SEARCH_DIRECT1, SEARCH_DIRECT7, SEARCH_DIRECT26, SEARCH_KDTREE, AlignResult, NdtError,
-                      NdtScanMatcher, NdtScanMatcherBuilder, NdtScanMatcherConfig, laplace_covariance_6x6,
+                      AlignServer, NdtScanMatcher, NdtScanMatcherBuilder, NdtScanMatcherConfig, laplace_covariance_6x6,
                       laplace_covariance_xy)
 
 __all__ = [

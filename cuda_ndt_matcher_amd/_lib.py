@@ -101,6 +101,7 @@ EXPORTS = [
     "ndt_hip_per_point_scores", "ndt_hip_evaluate_nvtl_batch", "ndt_hip_set_regularization_pose",
     "ndt_hip_clear_regularization_pose", "ndt_hip_laplace_covariance_xy", "ndt_hip_laplace_covariance_6x6",
     "ndt_hip_get_timing", "ndt_hip_get_phase_profile",
+    "ndt_hip_server_start", "ndt_hip_server_submit", "ndt_hip_server_poll", "ndt_hip_server_stop",
 ]
 
 _lib = None
@@ -162,6 +163,10 @@ def load():
     lib.ndt_hip_laplace_covariance_6x6.argtypes = [f64p, f64p]
     lib.ndt_hip_get_timing.argtypes = [vp, C.POINTER(NdtTiming)]
     lib.ndt_hip_get_phase_profile.argtypes = [vp, C.POINTER(NdtPhaseProfile)]
+    lib.ndt_hip_server_start.argtypes = [vp, C.c_int32, C.c_int32, sz, C.POINTER(vp)]
+    lib.ndt_hip_server_submit.argtypes = [vp, f32p, sz, f64p, C.POINTER(C.c_uint64)]
+    lib.ndt_hip_server_poll.argtypes = [vp, rp, C.POINTER(C.c_uint64), sz, C.POINTER(sz)]
+    lib.ndt_hip_server_stop.argtypes = [vp]
     _lib = lib
     return lib
 

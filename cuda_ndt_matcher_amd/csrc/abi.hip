@@ -648,3 +648,173 @@ int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Continuous-batching server (no counterpart in the reference's API: it
+// serves the multi-scan path, align_parallel_scans ndt.rs:1224-1320, as a
+// stream of requests instead of one batch).
+
+#include <atomic>
+#include <deque>
+
+struct ndt_server {
+  ndt_handle* h = nullptr;
+  int slots = 0, W = 0;
+  size_t max_points = 0;
+  ndt::ServerSlot* mb = nullptr;      // pinned host memory
+  ndt::ServerControl* ctl = nullptr;  // pinned host memory
+  float* host_points = nullptr;       // pinned host memory, slots x max_points x 3
+  DevBuf points, ws;
+  hipStream_t kstream = nullptr;
+  struct Pending {
+    uint64_t ticket;
+    std::vector<float> xyz;
+    double pose[6];
+  };
+  std::deque<Pending> queue;
+  std::vector<uint64_t> slot_ticket;  // 0: free
+  std::vector<unsigned long long> slot_seq;
+  uint64_t next_ticket = 1;
+  bool running = false;
+};
+
+namespace {
+
+template <class T>
+void store_release(T* p, T v) {
+  __atomic_store_n(p, v, __ATOMIC_RELEASE);
+}
+template <class T>
+T load_acquire(const T* p) {
+  return __atomic_load_n(p, __ATOMIC_ACQUIRE);
+}
+
+// Posts queued requests to free slots (host memory writes only: no runtime
+// call that could need the device while the server occupies it).
+int server_admit(ndt_server* s) {
+  for (int k = 0; k < s->slots && !s->queue.empty(); ++k) {
+    if (s->slot_ticket[k]) continue;
+    ndt_server::Pending& p = s->queue.front();
+    const size_t n = p.xyz.size() / 3;
+    std::memcpy(s->host_points + (size_t)k * s->max_points * 3, p.xyz.data(), n * 12);
+    s->mb[k].n = (uint32_t)n;
+    for (int i = 0; i < 6; ++i) s->mb[k].pose[i] = p.pose[i];
+    s->slot_seq[k] += 1;
+    store_release(&s->mb[k].seq, s->slot_seq[k]);
+    s->slot_ticket[k] = p.ticket;
+    s->queue.pop_front();
+  }
+  return NDT_OK;
+}
+
+}  // namespace
+
+int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size_t max_points, ndt_server** out) {
+  if (!h || !out || slots <= 0 || wg_per_slot <= 0 || max_points == 0 || max_points >= (1u << 31))
+    return set_err(h, NDT_ERR_INVALID_ARGUMENT, "invalid server shape");
+  *out = nullptr;
+  if (!h->grid.ready) return set_err(h, NDT_ERR_NO_TARGET, "No target point cloud set");
+  if ((size_t)slots * wg_per_slot > (size_t)std::max(1, h->coop_capacity))
+    return set_err(h, NDT_ERR_CAPACITY, "slots x workgroups exceed the co-resident workgroups");
+  auto* s = new ndt_server();
+  s->h = h;
+  s->slots = slots;
+  s->W = wg_per_slot;
+  s->max_points = max_points;
+  s->slot_ticket.assign(slots, 0);
+  s->slot_seq.assign(slots, 0);
+  auto fail = [&](hipError_t e, const char* what) {
+    ndt_hip_server_stop(s);
+    return hip_err(h, e, what);
+  };
+  hipError_t e;
+  const unsigned flags = hipHostMallocCoherent | hipHostMallocMapped;
+  if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->mb), sizeof(ndt::ServerSlot) * slots, flags)) != hipSuccess)
+    return fail(e, "server mailbox");
+  if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->ctl), sizeof(ndt::ServerControl), flags)) != hipSuccess)
+    return fail(e, "server control");
+  if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->host_points), (size_t)slots * max_points * 12, flags)) !=
+      hipSuccess)
+    return fail(e, "server points");
+  std::memset(s->mb, 0, sizeof(ndt::ServerSlot) * slots);
+  std::memset(s->ctl, 0, sizeof(ndt::ServerControl));
+  if ((e = s->points.reserve((size_t)slots * max_points * 12)) != hipSuccess) return fail(e, "server points");
+  if ((e = s->ws.reserve(ndt::server_workspace_bytes(slots, wg_per_slot))) != hipSuccess)
+    return fail(e, "server workspace");
+  if ((e = hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
+  const ndt::NewtonConfig cfg = newton_config(h, wg_per_slot);
+  // The mailbox and the workspace counters must be in place before the launch.
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "server setup");
+  if ((e = ndt::launch_server(h->grid.view, cfg, s->mb, s->ctl, slots, s->host_points,
+                              reinterpret_cast<float*>(s->points.ptr), (uint32_t)max_points, s->ws.ptr, s->kstream)) !=
+      hipSuccess)
+    return fail(e, "server launch");
+  s->running = true;
+  *out = s;
+  return NDT_OK;
+}
+
+int ndt_hip_server_submit(ndt_server* s, const float* xyz, size_t n, const double pose[6], uint64_t* ticket) {
+  if (!s || !ticket || (!xyz && n) || !pose) return NDT_ERR_INVALID_ARGUMENT;
+  ndt_handle* h = s->h;
+  if (n == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  if (n > s->max_points) return set_err(h, NDT_ERR_CAPACITY, "request exceeds the server's points per slot");
+  if (!s->running) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "server is not running");
+  ndt_server::Pending p;
+  p.ticket = s->next_ticket++;
+  p.xyz.assign(xyz, xyz + 3 * n);
+  for (int i = 0; i < 6; ++i) p.pose[i] = pose[i];
+  s->queue.push_back(std::move(p));
+  *ticket = s->queue.back().ticket;
+  return server_admit(s);
+}
+
+int ndt_hip_server_poll(ndt_server* s, ndt_result* results, uint64_t* tickets, size_t cap, size_t* count) {
+  if (!s || !count || (cap && (!results || !tickets))) return NDT_ERR_INVALID_ARGUMENT;
+  ndt_handle* h = s->h;
+  *count = 0;
+  for (int k = 0; k < s->slots && *count < cap; ++k) {
+    if (!s->slot_ticket[k]) continue;
+    if (load_acquire(&s->mb[k].done) != s->slot_seq[k]) continue;
+    results[*count] = s->mb[k].result;
+    tickets[*count] = s->slot_ticket[k];
+    ++*count;
+    s->slot_ticket[k] = 0;
+  }
+  int rc = server_admit(s);
+  if (rc) return rc;
+  bool busy = !s->queue.empty();
+  for (int k = 0; k < s->slots; ++k) busy |= s->slot_ticket[k] != 0;
+  if (busy && hipStreamQuery(s->kstream) == hipSuccess)
+    return set_err(h, NDT_ERR_HIP, "server kernel has exited (idle timeout); restart the server");
+  return NDT_OK;
+}
+
+int ndt_hip_server_stop(ndt_server* s) {
+  if (!s) return NDT_OK;
+  ndt_handle* h = s->h;
+  int rc = NDT_OK;
+  if (s->running) {
+    // Requests still on a slot finish first; queued ones are dropped.
+    s->queue.clear();
+    for (;;) {
+      bool busy = false;
+      for (int k = 0; k < s->slots; ++k)
+        if (s->slot_ticket[k] && load_acquire(&s->mb[k].done) != s->slot_seq[k]) busy = true;
+      if (!busy || hipStreamQuery(s->kstream) == hipSuccess) break;
+    }
+    store_release(&s->ctl->stop, 1u);
+    hipError_t e = hipStreamSynchronize(s->kstream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) rc = hip_err(h, e, "server kernel");
+    s->running = false;
+  }
+  if (s->kstream) (void)hipStreamDestroy(s->kstream);
+  s->points.release();
+  s->ws.release();
+  if (s->mb) (void)hipHostFree(s->mb);
+  if (s->ctl) (void)hipHostFree(s->ctl);
+  if (s->host_points) (void)hipHostFree(s->host_points);
+  delete s;
+  return rc;
+}

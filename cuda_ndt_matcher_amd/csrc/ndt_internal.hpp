@@ -100,6 +100,32 @@ static_assert(sizeof(ndt_result) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8, "ndt_result
 size_t newton_workspace_bytes(int slots, int wg_per_slot);
 hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
                          ndt_result* d_out, void* ws, hipStream_t st, bool cooperative, hipEvent_t ev_start);
+
+// Continuous-batching server.  The mailbox and the request points live in
+// pinned host memory: the host posts a job to a free slot by writing the
+// points, n and pose and then seq = previous + 1 with release semantics; the
+// slot's workgroups copy the points to device memory, run the alignment, and
+// workgroup 0 writes result and then done = seq.  The host sets stop only
+// when every slot is idle.
+struct alignas(128) ServerSlot {
+  unsigned long long seq;   // host: number of jobs posted to this slot
+  unsigned long long done;  // device: number of jobs finished
+  uint32_t n;
+  uint32_t pad;
+  double pose[6];
+  ndt_result result;
+};
+struct alignas(128) ServerControl {
+  uint32_t stop;
+};
+size_t server_workspace_bytes(int slots, int wg_per_slot);
+// Launches the persistent server kernel (cooperative): slots x wg_per_slot
+// workgroups.  host_points (pinned) and slot_points (device) each hold
+// slots x max_points x 3 floats.  A slot's workgroup 0 leaves the loop on
+// stop or after about 10 s without a job; the others follow it.
+hipError_t launch_server(const GridView& g, const NewtonConfig& cfg, ServerSlot* mailbox, ServerControl* ctl,
+                         int slots, const float* host_points, float* slot_points, uint32_t max_points, void* ws,
+                         hipStream_t st);
 int newton_max_coresident_workgroups();
 
 // One derivative pass (score, gradient, Hessian, count) at a pose.

@@ -587,11 +587,37 @@ __device__ uint32_t oscillation_count(const double* h, int k) {
   return mx;
 }
 
-// kF32: per-point terms in FP32 (the precision variant); FP64 otherwise.
+// Points [begin, last) of workgroup `rank` among the W of a slot.
+__device__ inline void slot_range(uint32_t n, int W, int rank, uint32_t& begin, uint32_t& last) {
+  const uint32_t per_wg = (n + (uint32_t)W - 1) / (uint32_t)W;
+  begin = min(n, (uint32_t)rank * per_wg);
+  last = min(n, begin + per_wg);
+}
+
+// Exchange state of workgroup `rank` of slot `slot`.
+__device__ inline SlotSync make_slot_sync(double* partials, uint32_t* counters, int slot, int W, int rank) {
+  SlotSync sy;
+  sy.partials = partials + (size_t)slot * 2 * W * kPartialStride;
+  uint32_t* lines = counters + (size_t)slot * counter_lines(W) * kCounterStride;
+  sy.top = lines;
+  sy.group = lines + (size_t)(1 + rank / kArrivalGroup) * kCounterStride;
+  sy.group_size = (uint32_t)min(kArrivalGroup, W - (rank / kArrivalGroup) * kArrivalGroup);
+  sy.groups = (uint32_t)((W + kArrivalGroup - 1) / kArrivalGroup);
+  sy.gen = 0;
+  sy.group_arrivals = 0;
+  sy.top_arrivals = 0;
+  sy.W = W;
+  sy.rank = rank;
+  return sy;
+}
+
+// One whole alignment by the W workgroups of a slot; this workgroup is
+// sy.rank.  Workgroup 0 writes *out.  kF32: per-point terms in FP32 (the
+// precision variant); FP64 otherwise.  The exchange state sy carries over
+// from one alignment to the next (the persistent server runs many).
 template <bool kF32>
-__global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig cfg, const SlotJob* __restrict__ jobs,
-                                                        ndt_result* __restrict__ out, double* __restrict__ partials,
-                                                        uint32_t* __restrict__ counters) {
+__device__ __attribute__((always_inline)) void align_slot(const GridView& g, const NewtonConfig& cfg,
+                                                        const SlotJob& job, ndt_result* out, SlotSync& sy) {
   __shared__ PoseTables s_tab;
   __shared__ SlotState S;
   __shared__ SolveScratch s_solve;
@@ -608,30 +634,13 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
   const Stage stage{s_big.rec, s_stage_slot};
   __shared__ double s_q[4];
 
-  const int W = cfg.wg_per_slot;
-  const int slot = blockIdx.x / W;
-  const int rank = blockIdx.x % W;
-  const SlotJob job = jobs[slot];
-  SlotSync sy;
-  sy.partials = partials + (size_t)slot * 2 * W * kPartialStride;
-  {
-    uint32_t* lines = counters + (size_t)slot * counter_lines(W) * kCounterStride;
-    sy.top = lines;
-    sy.group = lines + (size_t)(1 + rank / kArrivalGroup) * kCounterStride;
-    sy.group_size = (uint32_t)min(kArrivalGroup, W - (rank / kArrivalGroup) * kArrivalGroup);
-    sy.groups = (uint32_t)((W + kArrivalGroup - 1) / kArrivalGroup);
-  }
-  sy.gen = 0;
-  sy.group_arrivals = 0;
-  sy.top_arrivals = 0;
-  sy.W = W;
-  sy.rank = rank;
+  const int W = sy.W;
+  const int rank = sy.rank;
   // Workgroup `rank` owns points [begin, last); its thread t takes local
   // index (t % 64) * 4 + t / 64 and then steps by the block size, so a short
   // range is shared by all four waves (more loads in flight per CU).
-  const uint32_t per_wg = (job.n + (uint32_t)W - 1) / (uint32_t)W;
-  const uint32_t begin = min(job.n, (uint32_t)rank * per_wg);
-  const uint32_t last = min(job.n, begin + per_wg);
+  uint32_t begin, last;
+  slot_range(job.n, W, rank, begin, last);
   const uint32_t first = begin + (threadIdx.x % kWave) * (kBlock / kWave) + threadIdx.x / kWave;
   const uint32_t stride = kBlock;
   const int max_iter = cfg.max_iter < (uint32_t)kMaxIter ? (int)cfg.max_iter : kMaxIter;
@@ -880,7 +889,98 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
       for (int i = 0; i < 36; ++i) r.hessian[i] = S.final_H[i];
     }
     r.reserved = 0;
-    out[slot] = r;
+    *out = r;
+  }
+}
+
+template <bool kF32>
+__global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig cfg, const SlotJob* __restrict__ jobs,
+                                                        ndt_result* __restrict__ out, double* __restrict__ partials,
+                                                        uint32_t* __restrict__ counters) {
+  const int W = cfg.wg_per_slot;
+  const int slot = blockIdx.x / W;
+  SlotSync sy = make_slot_sync(partials, counters, slot, W, blockIdx.x % W);
+  const SlotJob job = jobs[slot];
+  align_slot<kF32>(g, cfg, job, out + slot, sy);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent server: each slot's workgroups run the jobs posted to the slot's
+// mailbox one after another (continuous batching: a slot that finishes takes
+// the next request while the others keep running).
+//
+// Workgroup 0 of a slot polls the host mailbox (bounded: it gives up after
+// kServerIdleSpins polls without a job) and announces its decision in the
+// slot's command word in device memory: the job number to run, or kExitCmd.
+// The other workgroups of the slot follow that word, so a slot never splits
+// between running a job and leaving.
+constexpr unsigned long long kServerIdleSpins = 1ull << 21;  // ~10 s at ~5 us per poll
+constexpr unsigned long long kExitCmd = ~0ull;
+
+// The host leaves each request's points in pinned memory (host_points); at
+// the start of a job every workgroup copies its own point range into the
+// slot's device buffer, which is all it reads afterwards.  No runtime copy
+// is needed while the launch occupies the device.
+template <bool kF32>
+__global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig cfg, ServerSlot* __restrict__ mb,
+                                                        const ServerControl* __restrict__ ctl,
+                                                        const float* __restrict__ host_points,
+                                                        float* __restrict__ slot_points, uint32_t max_points,
+                                                        double* __restrict__ partials, uint32_t* __restrict__ counters,
+                                                        unsigned long long* __restrict__ cmd) {
+  const int W = cfg.wg_per_slot;
+  const int slot = blockIdx.x / W;
+  const int rank = blockIdx.x % W;
+  SlotSync sy = make_slot_sync(partials, counters, slot, W, rank);
+  unsigned long long* my_cmd = cmd + (size_t)slot * kCounterStride / 2;
+  __shared__ unsigned long long s_cmd;
+  __shared__ SlotJob s_job;
+  unsigned long long done = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      unsigned long long c = kExitCmd;
+      if (rank == 0) {
+        for (unsigned long long spin = 0; spin < kServerIdleSpins; ++spin) {
+          if (__hip_atomic_load(&mb[slot].seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) > done) {
+            c = done + 1;
+            break;
+          }
+          if (__hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
+          __builtin_amdgcn_s_sleep(127);
+        }
+        __hip_atomic_store(my_cmd, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        // Workgroup 0 always writes the command word within its bounded poll.
+        for (;;) {
+          c = __hip_atomic_load(my_cmd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (c > done) break;
+          __builtin_amdgcn_s_sleep(8);
+        }
+      }
+      if (c != kExitCmd) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's job fields and points
+        s_job.pts = slot_points + (size_t)slot * max_points * 3;
+        s_job.n = mb[slot].n;
+        s_job.pad = 0;
+        for (int i = 0; i < 6; ++i) s_job.pose[i] = mb[slot].pose[i];
+      }
+      s_cmd = c;
+    }
+    __syncthreads();
+    if (s_cmd == kExitCmd) break;
+    const SlotJob job = s_job;
+    {
+      uint32_t begin, last;
+      slot_range(job.n, W, rank, begin, last);
+      const float* src = host_points + (size_t)slot * max_points * 3;
+      float* dst = slot_points + (size_t)slot * max_points * 3;
+      for (uint32_t k = 3 * begin + threadIdx.x; k < 3 * last; k += kBlock) dst[k] = src[k];
+    }
+    __syncthreads();
+    align_slot<kF32>(g, cfg, job, &mb[slot].result, sy);
+    ++done;
+    if (rank == 0 && threadIdx.x == 0)
+      __hip_atomic_store(&mb[slot].done, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -1017,13 +1117,19 @@ int newton_max_coresident_workgroups() {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  int per_cu32 = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(newton_kernel<false>),
-                                                   kBlock, 0) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu32, reinterpret_cast<const void*>(newton_kernel<true>),
-                                                   kBlock, 0) != hipSuccess)
-    return 0;
-  return cus * std::min(per_cu, per_cu32);
+  // The smallest over the persistent kernels (both precisions, one-shot and
+  // server), so any of them fits a launch sized by this.
+  const void* fns[] = {reinterpret_cast<const void*>(newton_kernel<false>),
+                       reinterpret_cast<const void*>(newton_kernel<true>),
+                       reinterpret_cast<const void*>(server_kernel<false>),
+                       reinterpret_cast<const void*>(server_kernel<true>)};
+  per_cu = 1 << 30;
+  for (const void* fn : fns) {
+    int k = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, fn, kBlock, 0) != hipSuccess) return 0;
+    per_cu = std::min(per_cu, k);
+  }
+  return cus * per_cu;
 }
 
 hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
@@ -1054,6 +1160,31 @@ hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJ
   else
     hipLaunchKernelGGL(newton_kernel<false>, grid, block, 0, st, g, cfg, d_jobs, d_out, partials, counters);
   return hipGetLastError();
+}
+
+size_t server_workspace_bytes(int slots, int wg_per_slot) {
+  return newton_workspace_bytes(slots, wg_per_slot) + (size_t)slots * kCounterStride * sizeof(uint32_t);
+}
+
+hipError_t launch_server(const GridView& g, const NewtonConfig& cfg, ServerSlot* mailbox, ServerControl* ctl,
+                         int slots, const float* host_points, float* slot_points, uint32_t max_points, void* ws,
+                         hipStream_t st) {
+  const int W = cfg.wg_per_slot;
+  double* partials = reinterpret_cast<double*>(ws);
+  uint32_t* counters =
+      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
+  const int nc = slots * counter_lines(W) * kCounterStride;
+  unsigned long long* cmd = reinterpret_cast<unsigned long long*>(counters + nc);
+  const int nall = nc + slots * kCounterStride;  // counters and command words
+  hipLaunchKernelGGL(zero_u32, dim3(std::min(1024, (nall + 255) / 256)), dim3(256), 0, st, counters, nall);
+  GridView gg = g;
+  NewtonConfig cc = cfg;
+  const float* hp = host_points;
+  float* pp = slot_points;
+  void* args[] = {&gg, &cc, &mailbox, &ctl, &hp, &pp, &max_points, &partials, &counters, &cmd};
+  const void* fn = cfg.precision == NDT_PRECISION_FP32 ? reinterpret_cast<const void*>(server_kernel<true>)
+                                                       : reinterpret_cast<const void*>(server_kernel<false>);
+  return hipLaunchCooperativeKernel(fn, dim3((unsigned)(slots * W)), dim3(kBlock), args, 0, st);
 }
 
 hipError_t launch_derivatives(const GridView& g, const NewtonConfig& cfg, const float* d_pts, uint32_t n,

@@ -342,6 +342,10 @@ class NdtScanMatcher:
         self._check(self._lib.ndt_hip_align_scans(self._h, ptrs, ns, poses, m, res))
         return [_result(res[i]) for i in range(m)]
 
+    def server(self, slots: int, workgroups_per_slot: int, max_points: int) -> "AlignServer":
+        """Continuous-batching server over this matcher's map (see AlignServer)."""
+        return AlignServer(self, slots, workgroups_per_slot, max_points)
+
     # -- scoring ---------------------------------------------------------------
     def evaluate_nvtl(self, source_points, pose) -> float:
         """evaluate_nvtl (ndt.rs:740-762)."""
@@ -410,3 +414,50 @@ def laplace_covariance_6x6(hessian) -> np.ndarray:
     if rc != 0:
         raise NdtError(rc, "laplace")
     return out.reshape(6, 6)
+
+
+class AlignServer:
+    """Continuous batching of alignments in one persistent launch.
+
+    ``slots`` requests run at once, each on ``workgroups_per_slot`` cooperating
+    workgroups; a request submitted while others run is posted to the first
+    slot that frees up.  ``submit`` returns a ticket; ``poll`` returns the
+    finished ``(ticket, AlignResult)`` pairs and admits queued requests.  A
+    result equals ``align`` of the same scan and pose with
+    ``workgroups_per_alignment = workgroups_per_slot``.  Use as a context
+    manager (``stop`` runs on exit); while it runs the matcher's other calls
+    wait for the device.
+    """
+
+    def __init__(self, matcher: "NdtScanMatcher", slots: int, workgroups_per_slot: int, max_points: int):
+        self._m = matcher
+        self._lib = matcher._lib
+        self._s = C.c_void_p()
+        self._cap = int(slots)
+        matcher._check(self._lib.ndt_hip_server_start(matcher._h, int(slots), int(workgroups_per_slot),
+                                                      int(max_points), C.byref(self._s)))
+        self._res = (_lib.NdtResult * self._cap)()
+        self._tk = (C.c_uint64 * self._cap)()
+
+    def submit(self, source_points, initial_guess) -> int:
+        pts = _points(source_points)
+        t = C.c_uint64()
+        self._m._check(self._lib.ndt_hip_server_submit(self._s, pts, pts.shape[0], _pose_vec(initial_guess),
+                                                       C.byref(t)))
+        return int(t.value)
+
+    def poll(self) -> List[Tuple[int, AlignResult]]:
+        cnt = C.c_size_t()
+        self._m._check(self._lib.ndt_hip_server_poll(self._s, self._res, self._tk, self._cap, C.byref(cnt)))
+        return [(int(self._tk[i]), _result(self._res[i])) for i in range(cnt.value)]
+
+    def stop(self) -> None:
+        if self._s:
+            s, self._s = self._s, C.c_void_p()
+            self._m._check(self._lib.ndt_hip_server_stop(s))
+
+    def __enter__(self) -> "AlignServer":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.stop()

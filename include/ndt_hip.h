@@ -218,6 +218,25 @@ int ndt_hip_clear_regularization_pose(ndt_handle* h);
 int ndt_hip_laplace_covariance_xy(const double hess[36], double cov[4]);
 int ndt_hip_laplace_covariance_6x6(const double hess[36], double cov[36]);
 
+/* Continuous-batching server.  No counterpart in the reference: it serves
+ * the multi-scan path (align_parallel_scans, ndt.rs:1224-1320) as a stream of
+ * requests.  One persistent launch holds `slots` slots of `wg_per_slot`
+ * workgroups; a request is posted to a free slot as soon as one frees up, so
+ * requests that arrive while others run are admitted without waiting for a
+ * batch to end.  submit copies the points (host) and queues the request;
+ * poll collects finished results (up to cap, with their tickets) and posts
+ * queued requests to free slots; stop lets running requests finish, drops
+ * queued ones and ends the launch.  Results equal an align_scans /
+ * ndt_hip_align of the same scan with workgroups_per_alignment = wg_per_slot.
+ * While a server runs it occupies every slot's workgroups: other calls on
+ * the handle wait for the GPU.  A slot left idle for about 10 s ends the
+ * launch; poll then reports NDT_ERR_HIP and the server must be restarted. */
+typedef struct ndt_server ndt_server;
+int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size_t max_points, ndt_server** out);
+int ndt_hip_server_submit(ndt_server* s, const float* xyz, size_t n, const double pose[6], uint64_t* ticket);
+int ndt_hip_server_poll(ndt_server* s, ndt_result* results, uint64_t* tickets, size_t cap, size_t* count);
+int ndt_hip_server_stop(ndt_server* s);
+
 int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out);
 int ndt_hip_get_phase_profile(const ndt_handle* h, ndt_phase_profile* out);
 
