@@ -8,6 +8,10 @@ scan already resident in HBM.  The map is built once on the GPU before timing
 Options: --gpus N (only 1 is supported: every run here uses one GPU),
 --steps K, --warmup W, --no-cpu (skip the CPU baseline), --particles P
 (also time the batched initial-pose search with P particles),
+--learned (train the bf16 proposal MLP on 16k scans of the 200k-point map
+and compare proposal-then-refine with the plain 200-particle search),
+--serve (continuous batching of a Poisson stream of 1k-40k-point requests
+after a closed-loop autotune of the slot shape: req/s, p50/p99 latency),
 --calibrate-copy MIB (run one device copy of MIB MiB before the timed
 region: a kernel of known traffic for scaling rocprofv3 FETCH_SIZE and
 WRITE_SIZE counters), --precision-sweep (the FP32 + two-sum and FP64 forms
@@ -108,6 +112,147 @@ def precision_sweep(seed, steps):
     return rows
 
 
+def serve_bench(m, wl, seed, n_requests=400):
+    """Continuous batching on the 5M-point map: requests of log-uniform 1k-40k
+    points (sub-sampled from a finer scan of the same lidar sweep), starts
+    N(0, 0.2 m) / N(0, 0.5 deg) around the true pose.  A closed-loop run picks
+    the (slots, workgroups per slot) shape with the best throughput; then a
+    Poisson stream at 80 % of that throughput is served and the latency from
+    arrival to result is recorded."""
+    from cuda_ndt_matcher_amd import synth
+
+    dense = synth.dense_scan("full", seed, 0.15)
+    rng = np.random.default_rng(seed + 17)
+    sizes = np.exp(rng.uniform(math.log(1000), math.log(40000), n_requests)).astype(int)
+    reqs = []
+    for n in sizes:
+        sel = np.sort(rng.choice(dense.shape[0], int(n), replace=False))
+        pose = wl.true_pose + np.concatenate([rng.normal(0, 0.2, 3), rng.normal(0, math.radians(0.5), 3)])
+        reqs.append((np.ascontiguousarray(dense[sel]), pose))
+
+    def closed_loop(slots, wg, count):
+        with m.server(slots, wg, 40000) as srv:
+            done, nxt = 0, 0
+            t0 = time.perf_counter()
+            while nxt < min(count, slots):
+                srv.submit(*reqs[nxt]); nxt += 1
+            while done < count:
+                for _ in srv.poll():
+                    done += 1
+                    if nxt < count:
+                        srv.submit(*reqs[nxt]); nxt += 1
+            return count / (time.perf_counter() - t0)
+
+    shapes = [(8, 32), (10, 25), (16, 16), (4, 64), (32, 8)]
+    tune = {f"{s}x{w}": closed_loop(s, w, 48) for s, w in shapes}
+    best = max(shapes, key=lambda sw: tune[f"{sw[0]}x{sw[1]}"])
+    rate = 0.8 * tune[f"{best[0]}x{best[1]}"]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    lat = np.zeros(n_requests)
+    with m.server(best[0], best[1], 40000) as srv:
+        t0 = time.perf_counter()
+        nxt, done, start = 0, 0, {}
+        while done < n_requests:
+            now = time.perf_counter() - t0
+            while nxt < n_requests and arrivals[nxt] <= now:
+                start[srv.submit(*reqs[nxt])] = nxt
+                nxt += 1
+            for t, _ in srv.poll():
+                k = start[t]
+                lat[k] = time.perf_counter() - t0 - arrivals[k]
+                done += 1
+        total = time.perf_counter() - t0
+    out = {"requests": n_requests, "sizes": "log-uniform 1k-40k points", "shape": f"{best[0]} slots x {best[1]} WG",
+           "autotune_req_per_s": tune, "offered_req_per_s": rate, "req_per_s": n_requests / total,
+           "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99))}
+    print(f"[bench] serve: {out['shape']}, {out['req_per_s']:.0f} req/s at {rate:.0f} offered, "
+          f"p50 {out['p50_ms']:.2f} ms, p99 {out['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+    return out
+
+
+def learned_bench(seed, n_train=196608, n_test=64, epochs=50, half=60.0):
+    """Learned proposal on the 200k-point map: train the MLP on n_train
+    seeded scans over [-half, half]^2 (labels: NDT poses converged from the
+    true poses; every batch re-rotates the scans to fresh headings), then
+    compare on n_test new scans the proposal search (16 hypotheses refined in
+    one launch, 200-particle fallback below NVTL 2.3) with the plain
+    200-particle search.  Success: <= 2 cm / 0.1 deg of the alignment
+    started at the true pose; agreement: the same bar against the plain
+    search's answer."""
+    import torch
+
+    from cuda_ndt_matcher_amd import NdtScanMatcher, proposal, synth
+
+    wl = synth.make_workload("small", seed)
+    m = NdtScanMatcher.builder().resolution(2.0).max_iterations(30).transformation_epsilon(0.01).build()
+    m.set_target(wl.map_points)
+    frame = proposal.Frame(0.0, 0.0, 4.0)
+    t0 = time.perf_counter()
+    store, labels, chunk = proposal.ScanStore(), [], 1024
+    for c in range(0, n_train, chunk):
+        part = proposal.make_samples(seed, seed * 7919 + 100 + c, min(chunk, n_train - c), half)
+        store.add([p.scan for p in part])
+        labels.append(proposal.refine_labels(m, part))
+    store.finish()
+    labels = np.concatenate(labels)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = proposal.ProposalNet(proposal.NET_SIZES, seed=seed)
+    t1 = time.perf_counter()
+    losses = proposal.train(net, store, labels, frame, epochs, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    train_points = int(store.xyz.shape[0])
+    del store
+    test = proposal.make_samples(seed, seed * 7919 + 999983, n_test, half)
+    ref = proposal.refine_labels(m, test)
+    tb = proposal.bev_batch([p.scan for p in test])
+
+    def close(a, b):
+        return (np.abs(a[:3] - b[:3]).max() <= 0.02
+                and np.rad2deg(np.abs(proposal.wrap(a[3:] - b[3:])).max()) <= 0.1)
+
+    # how close the best of the 16 raw hypotheses comes, before refinement
+    hyp = proposal.propose(net, frame, tb, proposal.HYPOTHESES)
+    dxy = np.hypot(hyp[..., 0] - ref[:, None, 0], hyp[..., 1] - ref[:, None, 1])
+    dyaw = np.rad2deg(np.abs(proposal.wrap(hyp[..., 2] - ref[:, None, 5])))
+    best_k = np.argmin(dxy / 1.0 + dyaw / 15.0, axis=1)
+    best_xy = dxy[np.arange(n_test), best_k]
+    best_yaw = dyaw[np.arange(n_test), best_k]
+    near = int(np.sum((best_xy < 1.0) & (best_yaw < 15.0)))
+    plain_ok, plain_t, learn_ok, learn_t, fallbacks, agree, direct = 0, 0.0, 0, 0.0, 0, 0, 0
+    for k, smp in enumerate(test):
+        t = time.perf_counter()
+        rp = proposal.plain_search(m, smp, 200, seed + k)
+        plain_t += time.perf_counter() - t
+        plain_ok += close(rp.pose_vector, ref[k])
+        t = time.perf_counter()
+        r, fb = proposal.learned_search(m, net, frame, smp, tb[k:k + 1], fallback=lambda s_, k=k: proposal.plain_search(
+            m, s_, 200, seed + k))
+        direct += (not fb) and close(r.pose_vector, ref[k])
+        learn_t += time.perf_counter() - t
+        learn_ok += close(r.pose_vector, ref[k])
+        agree += close(r.pose_vector, rp.pose_vector)
+        fallbacks += fb
+    out = {"map": "200k-point procedural map", "train_scans": n_train, "train_points": train_points,
+           "test_scans": n_test, "epochs": epochs, "region_m": 2 * half, "data_s": t_data, "train_s": t_train,
+           "loss_first_last": [losses[0], losses[-1]],
+           "plain": {"success_rate": plain_ok / n_test, "solves_per_s": n_test / plain_t},
+           "learned": {"success_rate": learn_ok / n_test, "solves_per_s": n_test / learn_t,
+                       "fallback_rate": fallbacks / n_test, "agrees_with_plain": agree / n_test,
+                       "success_without_fallback": direct / n_test},
+           "speedup": plain_t / learn_t,
+           "best_hypothesis_err_m_p50_p90": [float(np.percentile(best_xy, 50)), float(np.percentile(best_xy, 90))],
+           "best_hypothesis_err_deg_p50_p90": [float(np.percentile(best_yaw, 50)), float(np.percentile(best_yaw, 90))],
+           "within_1m_15deg": near}
+    print(f"[bench] learned proposal: loss {losses[0]:.3f} -> {losses[-1]:.3f} ({t_data:.1f} s data, {t_train:.1f} s "
+          f"training); plain {plain_ok}/{n_test} at {n_test / plain_t:.1f} solves/s, learned {learn_ok}/{n_test} at "
+          f"{n_test / learn_t:.1f} solves/s ({fallbacks} fallbacks, {direct} right without one, {agree} agree with plain); best hypothesis p50 "
+          f"{out['best_hypothesis_err_m_p50_p90'][0]:.2f} m {out['best_hypothesis_err_deg_p50_p90'][0]:.1f} deg, "
+          f"{near}/{n_test} within 1 m / 15 deg", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(wl, budget_s=12.0):
     from oracle import oracle as O
 
@@ -139,6 +284,10 @@ def main():
                     help="run one device copy of MIB MiB (counter calibration)")
     ap.add_argument("--precision-sweep", action="store_true",
                     help="also run the FP32/FP64 sweep over 5k-60k-point scans")
+    ap.add_argument("--learned", action="store_true",
+                    help="also train the proposal MLP and compare it with the plain particle search")
+    ap.add_argument("--serve", action="store_true",
+                    help="also serve a Poisson stream of 1k-40k-point requests (continuous batching)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -249,6 +398,10 @@ def main():
         "map_build_ms": build_ms,
         "map_build_gbs": build_bytes(wl.map_points.shape[0], voxels, 22) / (build_ms * 1e-3) / 1e9,
     }
+    if args.serve:
+        extra["serve"] = serve_bench(m, wl, args.seed)
+    if args.learned:
+        extra["learned_proposal"] = learned_bench(args.seed)
     if args.precision_sweep:
         extra["precision_sweep"] = precision_sweep(args.seed, max(3, args.steps // 5))
     line.update(extra)

@@ -102,6 +102,9 @@ EXPORTS = [
     "ndt_hip_clear_regularization_pose", "ndt_hip_laplace_covariance_xy", "ndt_hip_laplace_covariance_6x6",
     "ndt_hip_get_timing", "ndt_hip_get_phase_profile",
     "ndt_hip_server_start", "ndt_hip_server_submit", "ndt_hip_server_poll", "ndt_hip_server_stop",
+    "ndt_hip_mlp_create", "ndt_hip_mlp_destroy", "ndt_hip_mlp_last_error", "ndt_hip_mlp_num_params",
+    "ndt_hip_mlp_get_params", "ndt_hip_mlp_set_params", "ndt_hip_mlp_forward_device", "ndt_hip_mlp_forward_backward",
+    "ndt_hip_mlp_train_step_device", "ndt_hip_bev_device",
 ]
 
 _lib = None
@@ -167,6 +170,21 @@ def load():
     lib.ndt_hip_server_submit.argtypes = [vp, f32p, sz, f64p, C.POINTER(C.c_uint64)]
     lib.ndt_hip_server_poll.argtypes = [vp, rp, C.POINTER(C.c_uint64), sz, C.POINTER(sz)]
     lib.ndt_hip_server_stop.argtypes = [vp]
+    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    lib.ndt_hip_mlp_create.argtypes = [i32p, C.c_int32, C.c_uint64, C.POINTER(vp)]
+    lib.ndt_hip_mlp_destroy.argtypes = [vp]
+    lib.ndt_hip_mlp_destroy.restype = None
+    lib.ndt_hip_mlp_last_error.argtypes = [vp]
+    lib.ndt_hip_mlp_last_error.restype = C.c_char_p
+    lib.ndt_hip_mlp_num_params.argtypes = [vp]
+    lib.ndt_hip_mlp_num_params.restype = sz
+    lib.ndt_hip_mlp_get_params.argtypes = [vp, f32p]
+    lib.ndt_hip_mlp_set_params.argtypes = [vp, f32p]
+    lib.ndt_hip_mlp_forward_device.argtypes = [vp, vp, sz, f32p]
+    lib.ndt_hip_mlp_forward_backward.argtypes = [vp, f32p, sz, f32p, f32p, f32p]
+    lib.ndt_hip_mlp_train_step_device.argtypes = [vp, vp, f32p, sz, C.c_float, C.POINTER(C.c_float)]
+    lib.ndt_hip_bev_device.argtypes = [vp, vp, vp, sz, vp, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
+                                       vp]
     _lib = lib
     return lib
 

@@ -130,6 +130,16 @@ def perturb(pose: np.ndarray, seed: int, sigma_t: float, sigma_deg: float, rpy: 
 _SCENES = {}
 
 
+def dense_scan(name: str, seed: int, leaf: float) -> np.ndarray:
+    """The workload's lidar scan (same scene, sensor pose and beams)
+    downsampled at a finer `leaf`, for requests larger than the workload's."""
+    extent, nb, npl, _, beams, _, _, _, _ = WORKLOADS[name]
+    wl = make_workload(name, seed)
+    scene = Scene(seed, extent, nb, npl)
+    raw = scene.lidar(wl.true_pose[0], wl.true_pose[1], wl.true_pose[2], wl.true_pose[5], beams, seed=seed + 1)
+    return voxel_downsample(raw, leaf)
+
+
 def sweep_scans(seed: int, sizes, leaf: float = 0.1):
     """Scans of the given sizes over the small workload's map (the precision
     sweep): one 128-beam scan from the small workload's sensor pose,

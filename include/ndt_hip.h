@@ -237,6 +237,48 @@ int ndt_hip_server_submit(ndt_server* s, const float* xyz, size_t n, const doubl
 int ndt_hip_server_poll(ndt_server* s, ndt_result* results, uint64_t* tickets, size_t cap, size_t* count);
 int ndt_hip_server_stop(ndt_server* s);
 
+/* Learned initial-pose proposal.  No counterpart in the reference: it seeds
+ * the particle search of estimate_initial_pose (cuda_ndt_matcher/src/
+ * initial_pose.rs:115-400) with hypotheses from a multilayer perceptron over
+ * a bird's-eye-view descriptor of the scan.  Training and inference run on
+ * bf16 MFMA kernels with FP32 accumulation and FP32 master weights (Adam).
+ *
+ * sizes[0..n_sizes): layer widths, each a multiple of 32; hidden layers use
+ * GELU, the last is linear.  Parameters (get/set/gradients) are laid out per
+ * layer as W^T (out x in, row-major, f32) followed by the bias (out).  Batch
+ * sizes are multiples of 32.  train_step: one Adam step on the
+ * winner-takes-all proposal loss; the output holds H = out / 4 hypotheses
+ * (x, y, yaw, logit) and labels are (x, y, yaw) per sample. */
+typedef struct ndt_mlp ndt_mlp;
+int ndt_hip_mlp_create(const int32_t* sizes, int32_t n_sizes, uint64_t seed, ndt_mlp** out);
+void ndt_hip_mlp_destroy(ndt_mlp* m);
+const char* ndt_hip_mlp_last_error(const ndt_mlp* m);
+size_t ndt_hip_mlp_num_params(const ndt_mlp* m);
+int ndt_hip_mlp_get_params(ndt_mlp* m, float* params);
+int ndt_hip_mlp_set_params(ndt_mlp* m, const float* params);
+/* Forward of bf16 inputs in device memory (batch x sizes[0]); out: host, batch x sizes[last]. */
+int ndt_hip_mlp_forward_device(ndt_mlp* m, const uint16_t* d_x, size_t batch, float* out);
+/* Forward and backward of host inputs (rounded to bf16) with a given output
+ * gradient dout; returns the outputs and the parameter gradients. */
+int ndt_hip_mlp_forward_backward(ndt_mlp* m, const float* x, size_t batch, const float* dout, float* out,
+                                 float* grads);
+int ndt_hip_mlp_train_step_device(ndt_mlp* m, const uint16_t* d_x, const float* labels, size_t batch,
+                                  float learning_rate, float* mean_loss);
+/* Bird's-eye-view descriptors, one per output row j.  Row j describes scan
+ * q = d_index ? d_index[j] : j, whose points are d_xyz[offsets[q] ..
+ * offsets[q+1]) in the sensor frame.  When d_rot is not null the points are
+ * first rotated about z by (c, s) = (d_rot[2j], d_rot[2j+1]) (8-byte
+ * aligned): x' = c x - s y, y' = s x + c y, each product and difference
+ * rounded to f32 on its own.  A scan at map yaw t rotated by d reads as the
+ * same place at yaw t - d, which is how training re-labels a scan at many
+ * headings.  Output: log(1 + count) / 4 per cell of an nx x ny x nz grid
+ * (x/y cells of `cell` metres centred on the sensor, heights from zmin in
+ * steps of zcell), bf16, cell index (iy * nx + ix) * nz + iz.
+ * nx * ny * nz <= 32768. */
+int ndt_hip_bev_device(const float* d_xyz, const uint32_t* d_offsets, const uint32_t* d_index, size_t rows,
+                       const float* d_rot, int32_t nx, int32_t ny, int32_t nz, float cell, float zmin, float zcell,
+                       uint16_t* d_bev);
+
 int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out);
 int ndt_hip_get_phase_profile(const ndt_handle* h, ndt_phase_profile* out);
 
