@@ -972,7 +972,7 @@ void oracle_align(const oracle_grid* g, const float* xyz, size_t n, const double
     last_corr = d.corr;
     double rs, rg[6], rH[36];
     regularization(p, pose, d.corr, &rs, rg, rH);
-    const double score = d.score + rs;
+    double score = d.score + rs;
     double grad[6], H[36];
     for (int i = 0; i < 6; ++i) grad[i] = d.g[i] + rg[i];
     for (int i = 0; i < 6; ++i)
@@ -986,8 +986,20 @@ void oracle_align(const oracle_grid* g, const float* xyz, size_t n, const double
         x ^= x >> 31;
         return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
       };
-      for (int i = 0; i < 6; ++i) grad[i] *= 1.0 + p->noise * u(i);
-      for (int i = 0; i < 36; ++i) H[i] *= 1.0 + p->noise * u(64 + i);
+      // Rounding in a sum is relative to the size of its terms, not of the
+      // result: scale by the largest entry so that small (cancelled) entries
+      // move as much as rounding would move them.
+      double gmax = 0.0, hmax = 0.0;
+      for (int i = 0; i < 6; ++i) gmax = std::max(gmax, std::fabs(grad[i]));
+      for (int i = 0; i < 36; ++i) hmax = std::max(hmax, std::fabs(H[i]));
+      for (int i = 0; i < 6; ++i) grad[i] += p->noise * gmax * u(i);
+      for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {  // keep the Hessian symmetric
+          const double e = p->noise * hmax * u(64 + i * 6 + j);
+          H[i * 6 + j] += e;
+          if (j != i) H[j * 6 + i] += e;
+        }
+      score *= 1.0 + p->noise * u(128);  // the best-pose choice at max iterations compares scores
     }
     std::memcpy(last_H, H, sizeof(H));
     if (score > best_score) {  // solver.rs:189-193

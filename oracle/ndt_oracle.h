@@ -70,9 +70,11 @@ typedef struct oracle_params {
   int32_t has_regularization_pose;
   double regularization_translation[3];
   int32_t fp64_voxels;     /* 1 = use the FP64 voxel statistics (no f32 rounding) */
-  /* Conditioning probe (tests only): when > 0, every iteration's gradient and
-   * Hessian entries are scaled by 1 + noise*u, u in [-1, 1) from noise_seed,
-   * to measure how a run responds to rounding-level differences. */
+  /* Conditioning probe (tests only): when > 0, every iteration's score is
+   * scaled by 1 + noise*u and each gradient / Hessian entry moves by
+   * noise*u times the largest entry of its array (u in [-1, 1) from
+   * noise_seed; the Hessian stays symmetric), to measure how a run responds
+   * to rounding-level differences. */
   double noise;
   uint32_t noise_seed;
 } oracle_params;
