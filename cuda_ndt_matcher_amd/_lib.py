@@ -43,6 +43,8 @@ class NdtConfig(C.Structure):
         ("workgroups_per_alignment", C.c_int32),
         ("profile_phases", C.c_int32),
         ("cell_tables", C.c_int32),
+        ("use_graph", C.c_int32),
+        ("autotune", C.c_int32),
     ]
 
 
@@ -70,6 +72,17 @@ class NdtTiming(C.Structure):
         ("score_ms", C.c_float),
         ("align_workgroups", C.c_uint32),
         ("align_slots", C.c_uint32),
+        ("align_graph", C.c_uint32),
+        ("align_tuned", C.c_uint32),
+    ]
+
+
+class NdtTuningEntry(C.Structure):
+    _fields_ = [
+        ("scan_bits", C.c_int32),
+        ("batch_bits", C.c_int32),
+        ("workgroups_per_alignment", C.c_int32),
+        ("kernel_ms", C.c_float),
     ]
 
 
@@ -100,7 +113,7 @@ EXPORTS = [
     "ndt_hip_align_scans", "ndt_hip_evaluate_nvtl", "ndt_hip_evaluate_transform_probability",
     "ndt_hip_per_point_scores", "ndt_hip_evaluate_nvtl_batch", "ndt_hip_set_regularization_pose",
     "ndt_hip_clear_regularization_pose", "ndt_hip_laplace_covariance_xy", "ndt_hip_laplace_covariance_6x6",
-    "ndt_hip_get_timing", "ndt_hip_get_phase_profile",
+    "ndt_hip_get_timing", "ndt_hip_get_phase_profile", "ndt_hip_get_tuning",
     "ndt_hip_server_start", "ndt_hip_server_submit", "ndt_hip_server_poll", "ndt_hip_server_stop",
     "ndt_hip_mlp_create", "ndt_hip_mlp_destroy", "ndt_hip_mlp_last_error", "ndt_hip_mlp_num_params",
     "ndt_hip_mlp_get_params", "ndt_hip_mlp_set_params", "ndt_hip_mlp_forward_device", "ndt_hip_mlp_forward_backward",
@@ -165,6 +178,7 @@ def load():
     lib.ndt_hip_laplace_covariance_xy.argtypes = [f64p, f64p]
     lib.ndt_hip_laplace_covariance_6x6.argtypes = [f64p, f64p]
     lib.ndt_hip_get_timing.argtypes = [vp, C.POINTER(NdtTiming)]
+    lib.ndt_hip_get_tuning.argtypes = [vp, C.POINTER(NdtTuningEntry), sz, C.POINTER(sz)]
     lib.ndt_hip_get_phase_profile.argtypes = [vp, C.POINTER(NdtPhaseProfile)]
     lib.ndt_hip_server_start.argtypes = [vp, C.c_int32, C.c_int32, sz, C.POINTER(vp)]
     lib.ndt_hip_server_submit.argtypes = [vp, f32p, sz, f64p, C.POINTER(C.c_uint64)]

@@ -9,7 +9,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/ndt_hip.h"
@@ -37,6 +39,27 @@ struct ndt_handle {
   int coop_capacity = 0;
   double d1 = 0, d2 = 0;
   std::vector<int32_t> slot_to_index;  // voxel slot -> exported index (-1 if dropped)
+  ndt::SlotJob* host_jobs = nullptr;   // pinned staging for the jobs copy
+  size_t host_jobs_cap = 0;
+  // The last graph built for align* (use_graph) and what it was built for.
+  struct GraphCache {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t m = 0;
+    int W = 0;
+    const void *jobs = nullptr, *results = nullptr, *ws = nullptr, *hjobs = nullptr, *hres = nullptr;
+    ndt::NewtonConfig cfg;
+    ndt::GridView view;
+    void reset() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      exec = nullptr;
+      graph = nullptr;
+    }
+  } gcache;
+  // Autotuner: (bit length of the largest scan, bit length of the batch) ->
+  // (workgroups per alignment, kernel ms).
+  std::map<std::pair<int, int>, std::pair<int, float>> tuning;
 };
 
 namespace {
@@ -67,7 +90,8 @@ void gauss_params(double res, double outlier, double* d1, double* d2) {
 }
 
 ndt::NewtonConfig newton_config(const ndt_handle* h, int wg) {
-  ndt::NewtonConfig c{};
+  ndt::NewtonConfig c;
+  std::memset(&c, 0, sizeof(c));  // padding included: the graph cache compares bytes
   c.d1 = h->d1;
   c.d2 = h->d2;
   c.trans_eps = h->cfg.trans_epsilon;
@@ -122,6 +146,105 @@ int choose_wg(const ndt_handle* h, size_t n_max, size_t m) {
   return (int)std::max<size_t>(1, std::min<size_t>(want, (size_t)per_slot_cap));
 }
 
+int bit_length(size_t v) {
+  int b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
+// One launch of the Newton kernel over the m jobs staged in h->host_jobs,
+// with W workgroups per alignment; results land in h->host_results and the
+// kernel's device time in *ms.  With use_graph (and graph allowed), replays
+// the cached graph, rebuilding it when the shape, a buffer or the
+// configuration changed.
+int launch_once(ndt_handle* h, size_t m, int W, bool allow_graph, float* ms, bool* used_graph) {
+  NDT_HIP(h->ws.reserve(ndt::newton_workspace_bytes((int)m, W)), "alloc workspace");
+  const ndt::NewtonConfig cfg = newton_config(h, W);
+  const bool graph = allow_graph && h->cfg.use_graph && !h->cfg.profile_phases;
+  *used_graph = graph;
+  if (graph) {
+    ndt_handle::GraphCache& c = h->gcache;
+    const bool same = c.exec && c.m == m && c.W == W && c.jobs == h->jobs.ptr && c.results == h->results.ptr &&
+                      c.ws == h->ws.ptr && c.hjobs == h->host_jobs && c.hres == h->host_results &&
+                      std::memcmp(&c.cfg, &cfg, sizeof(cfg)) == 0 &&
+                      std::memcmp(&c.view, &h->grid.view, sizeof(c.view)) == 0;
+    if (!same) {
+      c.reset();
+      NDT_HIP(ndt::build_newton_graph(h->grid.view, cfg, reinterpret_cast<ndt::SlotJob*>(h->jobs.ptr), h->host_jobs,
+                                      (int)m, reinterpret_cast<ndt_result*>(h->results.ptr), h->host_results,
+                                      h->ws.ptr, h->ev0, h->ev1, &c.graph, &c.exec),
+              "build graph");
+      c.m = m;
+      c.W = W;
+      c.jobs = h->jobs.ptr;
+      c.results = h->results.ptr;
+      c.ws = h->ws.ptr;
+      c.hjobs = h->host_jobs;
+      c.hres = h->host_results;
+      std::memcpy(&c.cfg, &cfg, sizeof(cfg));
+      std::memcpy(&c.view, &h->grid.view, sizeof(c.view));
+    }
+    NDT_HIP(hipGraphLaunch(c.exec, h->stream), "graph launch");
+  } else {
+    if (h->cfg.profile_phases) {
+      NDT_HIP(h->phase.reserve(16 * sizeof(unsigned long long)), "alloc phase");
+      NDT_HIP(hipMemsetAsync(h->phase.ptr, 0, 16 * sizeof(unsigned long long), h->stream), "clear phase");
+    }
+    NDT_HIP(hipMemcpyAsync(h->jobs.ptr, h->host_jobs, m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
+            "copy jobs");
+    NDT_HIP(ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr), (int)m,
+                               reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream, W > 1, h->ev0),
+            "newton launch");
+    NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
+    NDT_HIP(hipMemcpyAsync(h->host_results, h->results.ptr, m * sizeof(ndt_result), hipMemcpyDeviceToHost,
+                           h->stream),
+            "copy results");
+  }
+  NDT_HIP(hipStreamSynchronize(h->stream), "newton sync");
+  NDT_HIP(hipEventElapsedTime(ms, h->ev0, h->ev1), "event time");
+  return NDT_OK;
+}
+
+// Autotuner: times each candidate workgroup count on the caller's own jobs
+// (one warm-up and two timed launches each, direct launches) and keeps the
+// fastest for the (scan size, batch size) bucket.  Candidates: the default
+// (about one point per thread) and the powers of two from an eighth of it to
+// four times it, all within what keeps every slot's workgroups co-resident.
+int autotune_wg(ndt_handle* h, size_t n_max, size_t m, int* W_out) {
+  const int cap = std::max(1, h->coop_capacity);
+  const int per_slot_cap = (int)std::max<size_t>(1, (size_t)cap / std::max<size_t>(1, m));
+  const int h0 = (int)std::max<size_t>(1, std::min<size_t>((n_max + 255) / 256, (size_t)per_slot_cap));
+  std::vector<int> cands = {h0};
+  for (int w = 1; w <= std::min(per_slot_cap, 4 * h0); w *= 2)
+    if (8 * w >= h0) cands.push_back(w);
+  std::sort(cands.begin(), cands.end());
+  cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+  int best_w = h0;
+  float best_ms = 3.0e38f;
+  for (int w : cands) {
+    float ms = 0.f, t = 0.f;
+    bool g = false;
+    int rc = launch_once(h, m, w, false, &ms, &g);
+    if (rc) return rc;
+    float tmin = 3.0e38f;
+    for (int r = 0; r < 2; ++r) {
+      rc = launch_once(h, m, w, false, &t, &g);
+      if (rc) return rc;
+      tmin = std::min(tmin, t);
+    }
+    if (tmin < best_ms) {
+      best_ms = tmin;
+      best_w = w;
+    }
+  }
+  h->tuning[{bit_length(n_max), bit_length(m)}] = {best_w, best_ms};
+  *W_out = best_w;
+  return NDT_OK;
+}
+
 // Runs m alignments: scan i = (d_pts[i], n[i]) from poses[6*i].
 int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, const double* poses, size_t m,
                    ndt_result* out) {
@@ -131,40 +254,50 @@ int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, co
     if (n[i] >= (1ull << 31)) return set_err(h, NDT_ERR_CAPACITY, "scan exceeds 2^31 points");
     n_max = std::max(n_max, n[i]);
   }
-  const int W = choose_wg(h, n_max, m);
-  if ((size_t)W * m > (size_t)std::max(1, h->coop_capacity) && W > 1)
-    return set_err(h, NDT_ERR_CAPACITY, "too many cooperating workgroups for one launch");
-  std::vector<ndt::SlotJob> jobs(m);
+  int rc = ensure_host_results(h, m);
+  if (rc) return rc;
+  if (m > h->host_jobs_cap) {
+    if (h->host_jobs) (void)hipHostFree(h->host_jobs);
+    h->host_jobs = nullptr;
+    h->host_jobs_cap = 0;
+    NDT_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->host_jobs), m * sizeof(ndt::SlotJob), hipHostMallocDefault),
+            "pinned alloc");
+    h->host_jobs_cap = m;
+  }
   for (size_t i = 0; i < m; ++i) {
-    jobs[i].pts = d_pts[i];
-    jobs[i].n = (uint32_t)n[i];
-    jobs[i].pad = 0;
-    for (int k = 0; k < 6; ++k) jobs[i].pose[k] = poses[6 * i + k];
+    ndt::SlotJob& j = h->host_jobs[i];
+    j.pts = d_pts[i];
+    j.n = (uint32_t)n[i];
+    j.pad = 0;
+    for (int k = 0; k < 6; ++k) j.pose[k] = poses[6 * i + k];
   }
   NDT_HIP(h->jobs.reserve(m * sizeof(ndt::SlotJob)), "alloc jobs");
   NDT_HIP(h->results.reserve(m * sizeof(ndt_result)), "alloc results");
-  NDT_HIP(h->ws.reserve(ndt::newton_workspace_bytes((int)m, W)), "alloc workspace");
-  if (h->cfg.profile_phases) {
-    NDT_HIP(h->phase.reserve(16 * sizeof(unsigned long long)), "alloc phase");
-    NDT_HIP(hipMemsetAsync(h->phase.ptr, 0, 16 * sizeof(unsigned long long), h->stream), "clear phase");
+  int W = choose_wg(h, n_max, m);
+  bool tuned = false;
+  if (h->cfg.autotune && h->cfg.workgroups_per_alignment <= 0 && !h->cfg.profile_phases) {
+    const auto it = h->tuning.find({bit_length(n_max), bit_length(m)});
+    if (it != h->tuning.end()) {
+      W = it->second.first;
+    } else {
+      rc = autotune_wg(h, n_max, m, &W);
+      if (rc) return rc;
+    }
+    // a bucket spans sizes up to twice its smallest: keep the slots co-resident
+    W = std::max(1, std::min(W, (int)std::max<size_t>(1, (size_t)std::max(1, h->coop_capacity) / m)));
+    tuned = true;
   }
-  int rc = ensure_host_results(h, m);
-  if (rc) return rc;
-  NDT_HIP(hipMemcpyAsync(h->jobs.ptr, jobs.data(), m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
-          "copy jobs");
-  const ndt::NewtonConfig cfg = newton_config(h, W);
-  NDT_HIP(ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr), (int)m,
-                             reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream, W > 1, h->ev0),
-          "newton launch");
-  NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
-  NDT_HIP(hipMemcpyAsync(h->host_results, h->results.ptr, m * sizeof(ndt_result), hipMemcpyDeviceToHost, h->stream),
-          "copy results");
-  NDT_HIP(hipStreamSynchronize(h->stream), "newton sync");
+  if ((size_t)W * m > (size_t)std::max(1, h->coop_capacity) && W > 1)
+    return set_err(h, NDT_ERR_CAPACITY, "too many cooperating workgroups for one launch");
   float ms = 0.f;
-  NDT_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time");
+  bool graph = false;
+  rc = launch_once(h, m, W, true, &ms, &graph);
+  if (rc) return rc;
   h->timing.align_ms = ms;
   h->timing.align_workgroups = (uint32_t)(W * m);
   h->timing.align_slots = (uint32_t)m;
+  h->timing.align_graph = graph ? 1u : 0u;
+  h->timing.align_tuned = tuned ? 1u : 0u;
   std::memcpy(out, h->host_results, m * sizeof(ndt_result));
   if (h->cfg.profile_phases) {
     unsigned long long ph[16];
@@ -265,6 +398,8 @@ void ndt_hip_config_default(ndt_config* c) {
   c->workgroups_per_alignment = 0;
   c->profile_phases = 0;
   c->cell_tables = 1;
+  c->use_graph = 0;
+  c->autotune = 0;
 }
 
 int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
@@ -306,8 +441,10 @@ void ndt_hip_destroy(ndt_handle* h) {
                     &G.valid, &G.table, &G.bitmap, &G.cell_nbr, &G.cell_ent, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
                     &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
+  h->gcache.reset();
   if (G.host_scalars) (void)hipHostFree(G.host_scalars);
   if (h->host_results) (void)hipHostFree(h->host_results);
+  if (h->host_jobs) (void)hipHostFree(h->host_jobs);
   for (hipEvent_t ev : {h->ev0, h->ev1, G.ev_build0, G.ev_build1, G.ev_sort0, G.ev_sort1})
     if (ev) (void)hipEventDestroy(ev);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -644,6 +781,22 @@ int ndt_hip_get_phase_profile(const ndt_handle* h, ndt_phase_profile* out) {
 int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out) {
   if (!h || !out) return NDT_ERR_INVALID_ARGUMENT;
   *out = h->timing;
+  return NDT_OK;
+}
+
+int ndt_hip_get_tuning(const ndt_handle* h, ndt_tuning_entry* out, size_t cap, size_t* count) {
+  if (!h || !count || (cap > 0 && !out)) return NDT_ERR_INVALID_ARGUMENT;
+  size_t k = 0;
+  for (const auto& e : h->tuning) {
+    if (k < cap) {
+      out[k].scan_bits = e.first.first;
+      out[k].batch_bits = e.first.second;
+      out[k].workgroups_per_alignment = e.second.first;
+      out[k].kernel_ms = e.second.second;
+    }
+    ++k;
+  }
+  *count = k;
   return NDT_OK;
 }
 

@@ -100,6 +100,14 @@ static_assert(sizeof(ndt_result) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8, "ndt_result
 size_t newton_workspace_bytes(int slots, int wg_per_slot);
 hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
                          ndt_result* d_out, void* ws, hipStream_t st, bool cooperative, hipEvent_t ev_start);
+// The launch_newton sequence as an instantiated graph: jobs copied from
+// pinned host h_jobs, counters filled by a kernel, ev0 recorded, the Newton
+// kernel (cooperative when wg_per_slot > 1), ev1 recorded, results copied to
+// pinned host h_out.  Every argument is fixed in the graph; replays read the
+// current contents of h_jobs.
+hipError_t build_newton_graph(const GridView& g, const NewtonConfig& cfg, SlotJob* d_jobs, const SlotJob* h_jobs,
+                              int slots, ndt_result* d_out, ndt_result* h_out, void* ws, hipEvent_t ev0,
+                              hipEvent_t ev1, hipGraph_t* graph, hipGraphExec_t* exec);
 
 // Continuous-batching server.  The mailbox and the request points live in
 // pinned host memory: the host posts a job to a free slot by writing the

@@ -1162,6 +1162,75 @@ hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJ
   return hipGetLastError();
 }
 
+// The same work as launch_newton, recorded once as a graph: jobs copy from
+// pinned host memory, counter fill (a kernel: no memset nodes), start event,
+// the persistent Newton kernel (cooperative attribute when W > 1), end event,
+// results copy to pinned host memory.  Replays read whatever h_jobs holds at
+// launch time, so a caller refills h_jobs and replays.
+hipError_t build_newton_graph(const GridView& g, const NewtonConfig& cfg, SlotJob* d_jobs, const SlotJob* h_jobs,
+                              int slots, ndt_result* d_out, ndt_result* h_out, void* ws, hipEvent_t ev0,
+                              hipEvent_t ev1, hipGraph_t* graph_out, hipGraphExec_t* exec_out) {
+  *graph_out = nullptr;
+  *exec_out = nullptr;
+  const int W = cfg.wg_per_slot;
+  double* partials = reinterpret_cast<double*>(ws);
+  uint32_t* counters =
+      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
+  int ncounters = slots * counter_lines(W) * kCounterStride;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipGraphCreate(&graph, 0);
+  if (e != hipSuccess) return e;
+  hipGraphNode_t n_jobs, n_zero, n_ev0, n_newton, n_ev1, n_res;
+  e = hipGraphAddMemcpyNode1D(&n_jobs, graph, nullptr, 0, d_jobs, h_jobs, (size_t)slots * sizeof(SlotJob),
+                              hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipKernelNodeParams kp{};
+    void* zargs[] = {&counters, &ncounters};
+    kp.func = reinterpret_cast<void*>(zero_u32);
+    kp.gridDim = dim3(std::min(1024, (ncounters + 255) / 256));
+    kp.blockDim = dim3(256);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = zargs;
+    kp.extra = nullptr;
+    e = hipGraphAddKernelNode(&n_zero, graph, &n_jobs, 1, &kp);
+  }
+  if (e == hipSuccess) e = hipGraphAddEventRecordNode(&n_ev0, graph, &n_zero, 1, ev0);
+  GridView gg = g;
+  NewtonConfig cc = cfg;
+  const SlotJob* jj = d_jobs;
+  ndt_result* oo = d_out;
+  void* args[] = {&gg, &cc, &jj, &oo, &partials, &counters};
+  if (e == hipSuccess) {
+    hipKernelNodeParams kp{};
+    kp.func = cfg.precision == NDT_PRECISION_FP32 ? reinterpret_cast<void*>(newton_kernel<true>)
+                                                  : reinterpret_cast<void*>(newton_kernel<false>);
+    kp.gridDim = dim3((unsigned)(slots * W));
+    kp.blockDim = dim3(kBlock);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    e = hipGraphAddKernelNode(&n_newton, graph, &n_ev0, 1, &kp);
+  }
+  if (e == hipSuccess && W > 1) {
+    hipKernelNodeAttrValue v{};
+    v.cooperative = 1;
+    e = hipGraphKernelNodeSetAttribute(n_newton, hipKernelNodeAttributeCooperative, &v);
+  }
+  if (e == hipSuccess) e = hipGraphAddEventRecordNode(&n_ev1, graph, &n_newton, 1, ev1);
+  if (e == hipSuccess)
+    e = hipGraphAddMemcpyNode1D(&n_res, graph, &n_ev1, 1, h_out, d_out, (size_t)slots * sizeof(ndt_result),
+                                hipMemcpyDeviceToHost);
+  hipGraphExec_t exec = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return e;
+  }
+  *graph_out = graph;
+  *exec_out = exec;
+  return hipSuccess;
+}
+
 size_t server_workspace_bytes(int slots, int wg_per_slot) {
   return newton_workspace_bytes(slots, wg_per_slot) + (size_t)slots * kCounterStride * sizeof(uint32_t);
 }

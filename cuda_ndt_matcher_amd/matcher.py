@@ -70,6 +70,8 @@ class NdtScanMatcherConfig:
     workgroups_per_alignment: int = 0
     profile_phases: bool = False
     cell_tables: bool = True
+    use_graph: bool = False
+    autotune: bool = False
 
     def to_c(self) -> _lib.NdtConfig:
         c = _lib.NdtConfig()
@@ -90,6 +92,8 @@ class NdtScanMatcherConfig:
         c.workgroups_per_alignment = int(self.workgroups_per_alignment)
         c.profile_phases = int(bool(self.profile_phases))
         c.cell_tables = int(bool(self.cell_tables))
+        c.use_graph = int(bool(self.use_graph))
+        c.autotune = int(bool(self.autotune))
         return c
 
 
@@ -156,6 +160,15 @@ class NdtScanMatcherBuilder:
         """Per-cell neighbour/slot tables for the lookups (identical results;
         hash + bitmap only when off)."""
         return self._set(cell_tables=v)
+
+    def use_graph(self, v: bool):
+        """Replay a HIP graph per launch shape for align* (same results)."""
+        return self._set(use_graph=v)
+
+    def autotune(self, v: bool):
+        """Pick workgroups per alignment by timing candidates once per
+        (scan size, batch size) bucket (with workgroups_per_alignment 0)."""
+        return self._set(autotune=v)
 
     def build(self) -> "NdtScanMatcher":
         return NdtScanMatcher.with_config(self.config)
@@ -397,6 +410,14 @@ class NdtScanMatcher:
         t = _lib.NdtTiming()
         self._check(self._lib.ndt_hip_get_timing(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def tuning(self) -> List[dict]:
+        """The autotuner's table: one entry per (scan_bits, batch_bits) bucket."""
+        cnt = C.c_size_t()
+        self._check(self._lib.ndt_hip_get_tuning(self._h, None, 0, C.byref(cnt)))
+        arr = (_lib.NdtTuningEntry * max(1, cnt.value))()
+        self._check(self._lib.ndt_hip_get_tuning(self._h, arr, cnt.value, C.byref(cnt)))
+        return [{k: getattr(arr[i], k) for k, _ in arr[i]._fields_} for i in range(cnt.value)]
 
 
 def laplace_covariance_xy(hessian) -> np.ndarray:

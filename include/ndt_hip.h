@@ -87,6 +87,12 @@ typedef struct ndt_config {
   int32_t cell_tables;                /* 1 = per-cell neighbour tables when the map's padded voxel box has
                                          at most 2^26 cells (20 bytes per cell); 0 = hash + bitmap lookups only.
                                          Both give the same neighbours in the same order. */
+  int32_t use_graph;                  /* 1 = align* calls replay a HIP graph per launch shape (jobs copy, counter
+                                         fill, persistent Newton kernel, results copy), rebuilt only when the
+                                         shape or a buffer changes; 0 = direct launches.  Same results. */
+  int32_t autotune;                   /* 1 = with workgroups_per_alignment 0, the first align* call of each
+                                         (scan size, batch size) bucket times candidate workgroup counts and
+                                         keeps the fastest for that bucket (see ndt_hip_get_tuning). */
 } ndt_config;
 
 /* Convergence status (optimization/types.rs:66-82). */
@@ -121,6 +127,8 @@ typedef struct ndt_timing {
   float score_ms;     /* last evaluate_*: the scoring kernel */
   uint32_t align_workgroups;
   uint32_t align_slots;
+  uint32_t align_graph;  /* 1 when the last align* replayed a graph */
+  uint32_t align_tuned;  /* 1 when its workgroup count came from the tuning table */
 } ndt_timing;
 
 /* Per-phase device time of the last align* call, from wall-clock stamps taken
@@ -280,6 +288,19 @@ int ndt_hip_bev_device(const float* d_xyz, const uint32_t* d_offsets, const uint
                        uint16_t* d_bev);
 
 int ndt_hip_get_timing(const ndt_handle* h, ndt_timing* out);
+
+/* Autotuner table (config autotune = 1; no counterpart in the reference):
+ * one entry per (scan size, batch size) bucket seen so far.  scan_bits is
+ * the bit length of the largest scan's point count, batch_bits that of the
+ * number of alignments in the call.  *count receives the number of entries;
+ * at most cap are written. */
+typedef struct ndt_tuning_entry {
+  int32_t scan_bits;
+  int32_t batch_bits;
+  int32_t workgroups_per_alignment;
+  float kernel_ms; /* best of two timed launches at that count */
+} ndt_tuning_entry;
+int ndt_hip_get_tuning(const ndt_handle* h, ndt_tuning_entry* out, size_t cap, size_t* count);
 int ndt_hip_get_phase_profile(const ndt_handle* h, ndt_phase_profile* out);
 
 #ifdef __cplusplus

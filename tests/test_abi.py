@@ -32,11 +32,36 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def test_struct_layouts():
+def test_struct_layouts(tmp_path):
+    """The ctypes mirrors have the header's sizes and field offsets (the
+    header compiled by gcc prints them)."""
+    import subprocess
+
     from cuda_ndt_matcher_amd import _lib
 
     assert C.sizeof(_lib.NdtResult) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8
-    assert C.sizeof(_lib.NdtConfig) == 88
+    mirrors = {"ndt_config": _lib.NdtConfig, "ndt_result": _lib.NdtResult, "ndt_timing": _lib.NdtTiming,
+               "ndt_phase_profile": _lib.NdtPhaseProfile, "ndt_tuning_entry": _lib.NdtTuningEntry}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ndt_hip.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            a, b, v = ln.split()
+            got[(a, b)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
 
 
 def test_config_defaults_match_reference():

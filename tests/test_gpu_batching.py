@@ -1,0 +1,117 @@
+"""Graph replay and the autotuner (GPU): both must leave the results alone.
+
+A graph replay runs the same kernel on the same inputs as a direct launch,
+so with the same workgroup count the results are bit-identical.  The
+autotuner only chooses the workgroup count; its answer equals a direct run
+at the count it chose.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(msg):
+    print(f"[case] {msg}", file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def small():
+    from cuda_ndt_matcher_amd import synth
+
+    return synth.make_workload("small", 0)
+
+
+def _matcher(points, **cfg):
+    from cuda_ndt_matcher_amd import NdtScanMatcher, NdtScanMatcherConfig
+
+    m = NdtScanMatcher(NdtScanMatcherConfig(**cfg))
+    m.set_target(points)
+    return m
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        assert np.array_equal(x.pose_vector, y.pose_vector)
+        assert x.iterations == y.iterations and x.status == y.status
+        assert x.score == y.score and x.nvtl == y.nvtl and x.transform_probability == y.transform_probability
+        assert np.array_equal(x.hessian, y.hessian)
+        assert x.num_correspondences == y.num_correspondences
+
+
+def _poses(small, seed, m):
+    rng = np.random.default_rng(seed)
+    return [small.true_pose + np.array([rng.normal(0, 0.5), rng.normal(0, 0.5), 0, 0, 0, rng.normal(0, 0.05)])
+            for _ in range(m)]
+
+
+@pytest.mark.parametrize("wg", [1, 4])
+def test_graph_replays_match_direct_launches(small, wg):
+    """Several replays with new poses (host work and other launches between
+    them), then a new batch size (the graph is rebuilt)."""
+    import torch
+
+    _print_case(f"graph batch wg={wg}")
+    direct = _matcher(small.map_points, workgroups_per_alignment=wg)
+    graph = _matcher(small.map_points, workgroups_per_alignment=wg, use_graph=True)
+    for rep, m in enumerate((16, 16, 16, 8, 16)):
+        poses = _poses(small, 100 + rep, m)
+        want = direct.align_batch(small.scan, poses)
+        got = graph.align_batch(small.scan, poses)
+        assert graph.timing()["align_graph"] == 1
+        _same(got, want)
+        # unrelated device work between replays
+        graph.evaluate_nvtl(small.scan, poses[0])
+        torch.ones(1024, device="cuda:0").sum().item()
+
+
+def test_graph_single_alignment_cooperative(small):
+    """One alignment over many cooperating workgroups (cooperative graph node)."""
+    _print_case("graph single")
+    direct = _matcher(small.map_points)
+    graph = _matcher(small.map_points, use_graph=True)
+    for k in range(3):
+        pose = _poses(small, 200 + k, 1)[0]
+        want = direct.align(small.scan, pose)
+        got = graph.align(small.scan, pose)
+        t = graph.timing()
+        assert t["align_graph"] == 1 and t["align_workgroups"] > 1
+        _same([got], [want])
+
+
+def test_graph_follows_a_new_map(small):
+    """set_target after a replay: the next call rebuilds against the new grid."""
+    from cuda_ndt_matcher_amd import synth
+
+    _print_case("graph new map")
+    other = synth.make_workload("small", 3)
+    graph = _matcher(small.map_points, use_graph=True, workgroups_per_alignment=2)
+    graph.align_batch(small.scan, _poses(small, 300, 4))
+    graph.set_target(other.map_points)
+    direct = _matcher(other.map_points, workgroups_per_alignment=2)
+    poses = [other.true_pose + d for d in np.random.default_rng(5).normal(0, 0.2, (4, 6)) * [1, 1, 0, 0, 0, 0.1]]
+    _same(graph.align_batch(other.scan, poses), direct.align_batch(other.scan, poses))
+
+
+def test_autotune_picks_a_count_and_keeps_it(small):
+    _print_case("autotune")
+    tuned = _matcher(small.map_points, autotune=True)
+    poses = _poses(small, 400, 8)
+    got = tuned.align_batch(small.scan, poses)
+    table = tuned.tuning()
+    assert len(table) == 1
+    e = table[0]
+    assert e["scan_bits"] == int(small.scan.shape[0]).bit_length() and e["batch_bits"] == (8).bit_length()
+    W = e["workgroups_per_alignment"]
+    assert W >= 1 and e["kernel_ms"] > 0
+    t = tuned.timing()
+    assert t["align_tuned"] == 1 and t["align_workgroups"] == 8 * W
+    _same(got, _matcher(small.map_points, workgroups_per_alignment=W).align_batch(small.scan, poses))
+    # same bucket: no new entry, same count
+    tuned.align_batch(small.scan, _poses(small, 401, 8))
+    assert tuned.tuning() == table and tuned.timing()["align_workgroups"] == 8 * W
+    # a new bucket gets its own entry
+    tuned.align(small.scan, poses[0])
+    assert len(tuned.tuning()) == 2
