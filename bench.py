@@ -113,6 +113,72 @@ def precision_sweep(seed, steps):
     return rows
 
 
+def batching_bench(wl, seed, reps=20):
+    """Launch modes on three shapes: one alignment of the 30k-point scan on
+    the 5M-point map; 200 particles (translation U(+-2 m), yaw U(+-pi)) of
+    the 1.5k-point scan on the 200k-point map; eight uneven scans
+    (log-uniform 1k-40k points) on the 5M-point map.  Modes: direct launches
+    at the default workgroup count, graph replay at that count, and the
+    autotuner (its first call tunes; timed calls reuse the table).  Reported:
+    host wall-clock per call (the caller's view, copies included) and the
+    kernel's device time.  Results are bit-identical between direct and graph
+    at one count."""
+    import torch
+
+    from cuda_ndt_matcher_amd import NdtScanMatcher, synth
+
+    small = synth.make_workload("small", seed)
+    rng = np.random.default_rng(seed + 31)
+    parts = [small.true_pose + np.array([rng.uniform(-2, 2), rng.uniform(-2, 2), 0.0, rng.normal(0, math.radians(1)),
+                                         rng.normal(0, math.radians(1)), rng.uniform(-math.pi, math.pi)])
+             for _ in range(200)]
+    dense = synth.dense_scan("full", seed, 0.15)
+    sizes = np.exp(rng.uniform(math.log(1000), math.log(40000), 8)).astype(int)
+    scans = [(np.ascontiguousarray(dense[np.sort(rng.choice(dense.shape[0], n, replace=False))]),
+              wl.true_pose + np.concatenate([rng.normal(0, 0.2, 3), rng.normal(0, math.radians(0.5), 3)]))
+             for n in sizes]
+    d_full = torch.from_numpy(wl.scan).to("cuda:0")
+    d_small = torch.from_numpy(small.scan).to("cuda:0")
+    torch.cuda.synchronize()
+    shapes = {
+        "single_30k": (wl.map_points, lambda m: m.align_device(d_full.data_ptr(), wl.scan.shape[0], wl.guess)),
+        "particles_200": (small.map_points,
+                          lambda m: m.align_batch_device(d_small.data_ptr(), small.scan.shape[0], parts)),
+        "multiscan_8": (wl.map_points, lambda m: m.align_parallel_scans(scans)),
+    }
+    out = {}
+    for name, (mp, call) in shapes.items():
+        row = {}
+        ref = None
+        for mode, kw in (("direct", {}), ("graph", {"use_graph": True}), ("autotune", {"autotune": True})):
+            b = NdtScanMatcher.builder()
+            for k, v in kw.items():
+                getattr(b, k)(v)
+            m = b.build()
+            m.set_target(mp)
+            r = call(m)
+            r = call(m)
+            kernel, t0 = [], time.perf_counter()
+            for _ in range(reps):
+                r = call(m)
+                kernel.append(m.timing()["align_ms"])
+            el = (time.perf_counter() - t0) / reps
+            res = r if isinstance(r, list) else [r]
+            if ref is None:
+                ref = res
+            t = m.timing()
+            row[mode] = {"ms_per_call": 1e3 * el, "kernel_ms": float(np.median(kernel)),
+                         "workgroups": int(t["align_workgroups"]), "slots": int(t["align_slots"]),
+                         "pose_equal_to_direct": all(np.array_equal(a.pose_vector, c.pose_vector) for a, c in zip(res, ref))}
+            if mode == "autotune":
+                row[mode]["table"] = m.tuning()
+        out[name] = row
+        print(f"[bench] batching {name}: " + ", ".join(
+            f"{k} {v['ms_per_call']:.3f} ms/call (kernel {v['kernel_ms']:.3f}, {v['workgroups']} WG)"
+            for k, v in row.items()), file=sys.stderr, flush=True)
+    return out
+
+
 def serve_bench(m, wl, seed, n_requests=400):
     """Continuous batching on the 5M-point map: requests of log-uniform 1k-40k
     points (sub-sampled from a finer scan of the same lidar sweep), starts
@@ -287,8 +353,12 @@ def main():
                     help="also run the FP32/FP64 sweep over 5k-60k-point scans")
     ap.add_argument("--learned", action="store_true",
                     help="also train the proposal MLP and compare it with the plain particle search")
+    ap.add_argument("--learned-train", type=int, default=196608, help="training scans for --learned")
+    ap.add_argument("--learned-epochs", type=int, default=50, help="training epochs for --learned")
     ap.add_argument("--serve", action="store_true",
                     help="also serve a Poisson stream of 1k-40k-point requests (continuous batching)")
+    ap.add_argument("--batching", action="store_true",
+                    help="also compare direct launches, graph replay and the autotuner on three launch shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -402,7 +472,9 @@ def main():
     if args.serve:
         extra["serve"] = serve_bench(m, wl, args.seed)
     if args.learned:
-        extra["learned_proposal"] = learned_bench(args.seed)
+        extra["learned_proposal"] = learned_bench(args.seed, n_train=args.learned_train, epochs=args.learned_epochs)
+    if args.batching:
+        extra["batching"] = batching_bench(wl, args.seed)
     if args.precision_sweep:
         extra["precision_sweep"] = precision_sweep(args.seed, max(3, args.steps // 5))
     line.update(extra)

@@ -77,6 +77,49 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void gemm_nt(const uint16_t* __re
     for (int j = 0; j < 4; ++j) C[(size_t)(tm + 8 * i + 4 * h + j) * N + tn + r] = acc[4 * i + j] + bb;
 }
 
+// Split-K form of gemm_nt for long K: grid (N / 128, M / 32, S); slice s
+// covers K range [s * kchunk, min(K, (s + 1) * kchunk)) and writes its
+// partial tile to P + s * M * N.  Two MFMA chains per wave (even and odd
+// 16-wide K steps) keep two products in flight.  kchunk and K are multiples
+// of 32.
+__global__ __launch_bounds__(64 * kWavesPerWg) void gemm_nt_splitk(const uint16_t* __restrict__ A,
+                                                                   const uint16_t* __restrict__ B,
+                                                                   float* __restrict__ P, int M, int N, int K,
+                                                                   int kchunk) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tn = (blockIdx.x * kWavesPerWg + wave) * kTile, tm = blockIdx.y * kTile;
+  if (tn >= N || tm >= M) return;
+  const int k0 = blockIdx.z * kchunk, k1 = min(K, k0 + kchunk);
+  const int r = lane & 31, h = lane >> 5;
+  const uint16_t* a = A + (size_t)(tm + r) * K + 8 * h;
+  const uint16_t* b = B + (size_t)(tn + r) * K + 8 * h;
+  floatx16 acc0 = {}, acc1 = {};
+  for (int k = k0; k < k1; k += 2 * kKStep) {
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(a + k);
+    const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(b + k);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(a + k + kKStep);
+    const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(b + k + kKStep);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc1, 0, 0, 0);
+  }
+  float* out = P + (size_t)blockIdx.z * M * N;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      out[(size_t)(tm + 8 * i + 4 * h + j) * N + tn + r] = acc0[4 * i + j] + acc1[4 * i + j];
+}
+
+// C[i] = sum over s = 0 .. S-1 of P[s * MN + i] (in that order) + bias[i % N].
+__global__ void splitk_reduce(const float* __restrict__ P, float* __restrict__ C, const float* __restrict__ bias,
+                              size_t MN, int N, int S) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < MN; i += (size_t)gridDim.x * blockDim.x) {
+    float v = 0.0f;
+    for (int s = 0; s < S; ++s) v += P[(size_t)s * MN + i];
+    C[i] = bias ? v + bias[i % N] : v;
+  }
+}
+
 __device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float z) {
   const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
@@ -127,16 +170,26 @@ __global__ void to_bf16(const float* __restrict__ in, uint16_t* __restrict__ out
 }
 
 // db[n] = sum over rows of dz (R x N), fixed order.
-__global__ void column_sums(const float* __restrict__ dz, float* __restrict__ db, int R, int N) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  float s = 0.0f;
-  for (int r = 0; r < R; ++r) s += dz[(size_t)r * N + n];
-  db[n] = s;
+// db[c] = sum over rows of dz[r][c] (R x N row-major).  Workgroup of 1024
+// threads per 64 columns: thread (c, g) sums rows g, g + 16, ... of column c,
+// then the 16 partials of a column are added in order g = 0 .. 15.
+__global__ __launch_bounds__(1024) void column_sums(const float* __restrict__ dz, float* __restrict__ db, int R,
+                                                    int N) {
+  __shared__ float part[16][64];
+  const int cl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
+  float v = 0.0f;
+  if (c < N)
+    for (int r = g; r < R; r += 16) v += dz[(size_t)r * N + c];
+  part[g][cl] = v;
+  __syncthreads();
+  if (g == 0 && c < N) {
+    float t = 0.0f;
+    for (int k = 0; k < 16; ++k) t += part[k][cl];
+    db[c] = t;
+  }
 }
 
-// Adam on f32 masters; refreshes the bf16 W^T copy (same layout as the
-// master) and, when wcopy is set, the bf16 W copy (transposed).
 __global__ void adam_step(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
                           float* __restrict__ v, uint16_t* __restrict__ wt_bf, size_t n, float lr, float b1, float b2,
                           float c1, float c2, float eps) {
@@ -245,6 +298,8 @@ struct ndt_mlp {
   float *dz = nullptr, *dy = nullptr, *dout = nullptr, *loss = nullptr, *label = nullptr;
   uint16_t *dz_bf = nullptr, *dzt_bf = nullptr, *actt_bf = nullptr;
   float* out_host = nullptr;
+  float* splitk = nullptr;  // split-K partial tiles
+  size_t splitk_cap = 0;    // floats
   long step = 0;
   std::vector<void*> allocs;
 };
@@ -266,6 +321,26 @@ hipError_t mlp_alloc(ndt_mlp* m, T** p, size_t n) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T));
   if (e == hipSuccess) m->allocs.push_back(*p);
   return e;
+}
+
+// Split-K plan for an M x N x K product: slices of kchunk (a multiple of 32)
+// so that the tiles times the slices reach about 2048 waves; one slice when
+// K is short or the tiles already fill the GPU.
+constexpr int kSplitMinK = 4096;
+void splitk_plan(int M, int N, int K, int* S, int* kchunk) {
+  const int tiles = (M / kTile) * (N / kTile);
+  int s = 1;
+  if (K >= kSplitMinK && tiles < 2048) s = std::min((2048 + tiles - 1) / tiles, K / 512);
+  int kc = (K + s - 1) / s;
+  kc = (kc + 2 * kKStep - 1) / (2 * kKStep) * (2 * kKStep);
+  *kchunk = kc;
+  *S = (K + kc - 1) / kc;
+}
+
+size_t splitk_floats(int M, int N, int K) {
+  int S, kc;
+  splitk_plan(M, N, K, &S, &kc);
+  return S > 1 ? (size_t)S * M * N : 0;
 }
 
 int ensure_batch(ndt_mlp* m, int B) {
@@ -293,12 +368,31 @@ int ensure_batch(ndt_mlp* m, int B) {
   MLP_HIP(hipMalloc(&m->dzt_bf, (size_t)B * widest * 2), "alloc dz");
   MLP_HIP(hipMalloc(&m->actt_bf, (size_t)B * widest * 2), "alloc act");
   MLP_HIP(hipHostMalloc(&m->out_host, (size_t)B * std::max<size_t>(m->sizes.back(), 1) * 4), "alloc out");
+  // split-K partials: the forward products (B x N x K) are the long-K ones
+  size_t sk = 0;
+  for (int l = 0; l < m->layers; ++l) sk = std::max(sk, splitk_floats(B, m->sizes[l + 1], m->sizes[l]));
+  if (sk > m->splitk_cap) {
+    if (m->splitk) (void)hipFree(m->splitk);
+    m->splitk = nullptr;
+    m->splitk_cap = 0;
+    MLP_HIP(hipMalloc(&m->splitk, sk * 4), "alloc split-K");
+    m->splitk_cap = sk;
+  }
   m->cap = B;
   return NDT_OK;
 }
 
 void launch_gemm(ndt_mlp* m, const uint16_t* A, const uint16_t* B, float* C, const float* bias, int M, int N, int K) {
   const dim3 grid((unsigned)((N + kTile * kWavesPerWg - 1) / (kTile * kWavesPerWg)), (unsigned)(M / kTile));
+  int S, kc;
+  splitk_plan(M, N, K, &S, &kc);
+  if (S > 1 && (size_t)S * M * N <= m->splitk_cap && K % (2 * kKStep) == 0) {
+    hipLaunchKernelGGL(gemm_nt_splitk, dim3(grid.x, grid.y, (unsigned)S), dim3(64 * kWavesPerWg), 0, m->st, A, B,
+                       m->splitk, M, N, K, kc);
+    const size_t mn = (size_t)M * N;
+    hipLaunchKernelGGL(splitk_reduce, dim3(grid1(mn)), dim3(256), 0, m->st, m->splitk, C, bias, mn, N, S);
+    return;
+  }
   hipLaunchKernelGGL(gemm_nt, grid, dim3(64 * kWavesPerWg), 0, m->st, A, B, C, bias, M, N, K, bias ? kBias : kStore);
 }
 
@@ -326,7 +420,7 @@ void backward(ndt_mlp* m, int B) {
       dz = m->dz;
     }
     // gb = column sums of dz; gw (N x K) = dz^T act[l]
-    hipLaunchKernelGGL(column_sums, dim3((N + 255) / 256), dim3(256), 0, m->st, dz, m->gb[l], B, N);
+    hipLaunchKernelGGL(column_sums, dim3((N + 63) / 64), dim3(1024), 0, m->st, dz, m->gb[l], B, N);
     hipLaunchKernelGGL(transpose_to_bf16<float>, dim3((N + 31) / 32, (B + 31) / 32), dim3(256), 0, m->st, dz,
                        m->dzt_bf, B, N);
     hipLaunchKernelGGL(transpose_to_bf16<uint16_t>, dim3((K + 31) / 32, (B + 31) / 32), dim3(256), 0, m->st,
@@ -424,6 +518,7 @@ void ndt_hip_mlp_destroy(ndt_mlp* m) {
                   (void*)m->dzt_bf, (void*)m->actt_bf})
     if (p) (void)hipFree(p);
   if (m->out_host) (void)hipHostFree(m->out_host);
+  if (m->splitk) (void)hipFree(m->splitk);
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
 }
