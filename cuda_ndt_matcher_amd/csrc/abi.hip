@@ -162,7 +162,8 @@ int bit_length(size_t v) {
 // configuration changed.
 int launch_once(ndt_handle* h, size_t m, int W, bool allow_graph, float* ms, bool* used_graph) {
   NDT_HIP(h->ws.reserve(ndt::newton_workspace_bytes((int)m, W)), "alloc workspace");
-  const ndt::NewtonConfig cfg = newton_config(h, W);
+  if (h->cfg.profile_phases) NDT_HIP(h->phase.reserve(16 * sizeof(unsigned long long)), "alloc phase");
+  const ndt::NewtonConfig cfg = newton_config(h, W);  // after the reserves: it carries their pointers
   const bool graph = allow_graph && h->cfg.use_graph && !h->cfg.profile_phases;
   *used_graph = graph;
   if (graph) {
@@ -189,10 +190,8 @@ int launch_once(ndt_handle* h, size_t m, int W, bool allow_graph, float* ms, boo
     }
     NDT_HIP(hipGraphLaunch(c.exec, h->stream), "graph launch");
   } else {
-    if (h->cfg.profile_phases) {
-      NDT_HIP(h->phase.reserve(16 * sizeof(unsigned long long)), "alloc phase");
+    if (h->cfg.profile_phases)
       NDT_HIP(hipMemsetAsync(h->phase.ptr, 0, 16 * sizeof(unsigned long long), h->stream), "clear phase");
-    }
     NDT_HIP(hipMemcpyAsync(h->jobs.ptr, h->host_jobs, m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
             "copy jobs");
     NDT_HIP(ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr), (int)m,

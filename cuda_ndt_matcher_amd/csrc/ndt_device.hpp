@@ -742,6 +742,10 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
     bool fast_exit = false;
     for (int sweep = 0; sweep < 60; ++sweep) {
       double off = (i < j) ? aij * aij : 0.0, dg = (live && i == j) ? aij * aij : 0.0;
+      // The sweep's first round runs on copies while the convergence sums
+      // travel through the butterfly; it is kept only if the sweep goes on.
+      double a0 = aij, v0 = vij;
+      jacobi_round<0>(a0, v0, i, j, live, lane);
 #pragma unroll
       for (int o = 32; o >= 1; o >>= 1) {
         off += __shfl_xor(off, o, kWave);
@@ -759,7 +763,8 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
         break;
       }
       if (lane == 0) ++W.sweeps;
-      jacobi_round<0>(aij, vij, i, j, live, lane);
+      aij = a0;
+      vij = v0;
       jacobi_round<1>(aij, vij, i, j, live, lane);
       jacobi_round<2>(aij, vij, i, j, live, lane);
       jacobi_round<3>(aij, vij, i, j, live, lane);

@@ -115,3 +115,12 @@ def test_autotune_picks_a_count_and_keeps_it(small):
     # a new bucket gets its own entry
     tuned.align(small.scan, poses[0])
     assert len(tuned.tuning()) == 2
+
+
+def test_phase_profile_on_the_first_call(small):
+    """The phase buffer exists before the first launch reads its address."""
+    _print_case("phase profile first call")
+    m = _matcher(small.map_points, profile_phases=True)
+    r = m.align(small.scan, small.guess)
+    ph = m.phase_profile()
+    assert ph["iterations"] == r.iterations and ph["total_us"] > 0 and ph["derivative_us"] > 0
