@@ -71,7 +71,26 @@ __device__ inline void block_minmax_commit(int32_t mn[3], int32_t mx[3], int32_t
 __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restrict__ pts, size_t n, float res,
                                                           int32_t* bounds /* minx,miny,minz,maxx,maxy,maxz */) {
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const size_t S = (size_t)gridDim.x * blockDim.x;
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  // Four points per step, all loads issued before any key: a streaming pass
+  // is bound by how many loads each wave keeps in flight.
+  for (; i + 3 * S < n; i += 4 * S) {
+    float v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[u][a] = pts[3 * (i + u * S) + a];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int32_t k = voxel_key(v[u][a], res);
+        mn[a] = min(mn[a], k);
+        mx[a] = max(mx[a], k);
+      }
+  }
+  for (; i < n; i += S) {
     for (int a = 0; a < 3; ++a) {
       const int32_t k = voxel_key(pts[3 * i + a], res);
       mn[a] = min(mn[a], k);
@@ -108,14 +127,26 @@ __global__ __launch_bounds__(kBuildBlock) void keygen_hist(const float* __restri
   __shared__ uint32_t s_hist[8 * kRadix];
   for (int i = threadIdx.x; i < kp.passes * kRadix; i += kBuildBlock) s_hist[i] = 0;
   __syncthreads();
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t rx = (uint32_t)(voxel_key(pts[3 * i + 0], kp.res) - kp.kmin[0]);
-    const uint32_t ry = (uint32_t)(voxel_key(pts[3 * i + 1], kp.res) - kp.kmin[1]);
-    const uint32_t rz = (uint32_t)(voxel_key(pts[3 * i + 2], kp.res) - kp.kmin[2]);
+  auto one = [&](size_t i, float x, float y, float z) {
+    const uint32_t rx = (uint32_t)(voxel_key(x, kp.res) - kp.kmin[0]);
+    const uint32_t ry = (uint32_t)(voxel_key(y, kp.res) - kp.kmin[1]);
+    const uint32_t rz = (uint32_t)(voxel_key(z, kp.res) - kp.kmin[2]);
     const uint64_t m = compact_morton(rx, ry, rz, kp.bits[0], kp.bits[1], kp.bits[2]);
     keys[i] = (K)m;
     for (int p = 0; p < kp.passes; ++p) atomicAdd(&s_hist[p * kRadix + ((m >> (8 * p)) & 0xFF)], 1u);
+  };
+  const size_t S = (size_t)gridDim.x * blockDim.x;
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (; i + 3 * S < n; i += 4 * S) {  // four points per step, loads first
+    float v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[u][a] = pts[3 * (i + u * S) + a];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) one(i + u * S, v[u][0], v[u][1], v[u][2]);
   }
+  for (; i < n; i += S) one(i, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
   __syncthreads();
   for (int i = threadIdx.x; i < kp.passes * kRadix; i += kBuildBlock)
     if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
@@ -374,12 +405,30 @@ __global__ __launch_bounds__(kBuildBlock) void voxel_stats(const float* __restri
   if (cnt < sp.min_points) return;
   const double ox = p0x, oy = p0y, oz = p0z;
   double sx = 0, sy = 0, sz = 0, sxx = 0, sxy = 0, sxz = 0, syy = 0, syz = 0, szz = 0;
-  for (uint32_t j = b; j < e; ++j) {
-    const uint32_t i = order[j];
-    const double x = (double)pts[3 * i] - ox, y = (double)pts[3 * i + 1] - oy, z = (double)pts[3 * i + 2] - oz;
+  auto add = [&](float px, float py, float pz) {
+    const double x = (double)px - ox, y = (double)py - oy, z = (double)pz - oz;
     sx += x; sy += y; sz += z;
     sxx += x * x; sxy += x * y; sxz += x * z;
     syy += y * y; syz += y * z; szz += z * z;
+  };
+  uint32_t j = b;
+  // Four points per step with every load issued first (the gathers dominate);
+  // the sums still run in point order.
+  for (; j + 4 <= e; j += 4) {
+    uint32_t id[4];
+    float v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) id[u] = order[j + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[u][a] = pts[3 * id[u] + a];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) add(v[u][0], v[u][1], v[u][2]);
+  }
+  for (; j < e; ++j) {
+    const uint32_t i = order[j];
+    add(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
   }
   const double n = (double)cnt;
   const double mx = sx / n, my = sy / n, mz = sz / n;
@@ -603,7 +652,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   int32_t* bounds = reinterpret_cast<int32_t*>(G.scalars.ptr) + 8;
   uint32_t* extra = reinterpret_cast<uint32_t*>(G.scalars.ptr);
   hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra, 8);
-  hipLaunchKernelGGL(key_bounds, dim3(grid_for(n, kBuildBlock, 512)), dim3(kBuildBlock), 0, st, G.points, n, res,
+  hipLaunchKernelGGL(key_bounds, dim3(grid_for(n, kBuildBlock, 1024)), dim3(kBuildBlock), 0, st, G.points, n, res,
                      bounds);
   if ((e = hipMemcpyAsync(G.host_scalars + 8, bounds, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
     return fail(e, "bounds copy");
