@@ -39,6 +39,7 @@ struct ndt_handle {
   int coop_capacity = 0;
   double d1 = 0, d2 = 0;
   std::vector<int32_t> slot_to_index;  // voxel slot -> exported index (-1 if dropped)
+  ndt_server* server = nullptr;        // the running continuous-batching server, if any
   ndt::SlotJob* host_jobs = nullptr;   // pinned staging for the jobs copy
   size_t host_jobs_cap = 0;
   // The last graph built for align* (use_graph) and what it was built for.
@@ -109,8 +110,19 @@ ndt::NewtonConfig newton_config(const ndt_handle* h, int wg) {
   return c;
 }
 
+// A running server holds every co-resident workgroup of the device, so any
+// other launch from this handle would queue behind it while the caller still
+// has to poll it: refused instead.
+int check_no_server(ndt_handle* h) {
+  if (h->server)
+    return set_err(h, NDT_ERR_INVALID_ARGUMENT,
+                   "a continuous-batching server is running on this handle; stop it first");
+  return NDT_OK;
+}
+
 int check_ready(ndt_handle* h) {
   if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  if (int rc = check_no_server(h)) return rc;
   if (!h->grid.ready) return set_err(h, NDT_ERR_NO_TARGET, "No target set. Call set_target() first.");
   NDT_HIP(hipSetDevice(h->device), "hipSetDevice");
   return NDT_OK;
@@ -432,6 +444,7 @@ int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
 
 void ndt_hip_destroy(ndt_handle* h) {
   if (!h) return;
+  if (h->server) (void)ndt_hip_server_stop(h->server);
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   ndt::DeviceGrid& G = h->grid;
@@ -454,6 +467,7 @@ const char* ndt_hip_last_error(const ndt_handle* h) { return h ? h->err.c_str() 
 
 int ndt_hip_set_target(ndt_handle* h, const float* xyz, size_t n) {
   if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  if (int rc = check_no_server(h)) return rc;
   if (n == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Target point cloud is empty");
   if (!xyz) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null points");
   NDT_HIP(hipSetDevice(h->device), "hipSetDevice");
@@ -466,6 +480,7 @@ int ndt_hip_set_target(ndt_handle* h, const float* xyz, size_t n) {
 
 int ndt_hip_set_target_device(ndt_handle* h, const float* d_xyz, size_t n) {
   if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  if (int rc = check_no_server(h)) return rc;
   if (n == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Target point cloud is empty");
   if (!d_xyz) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null points");
   NDT_HIP(hipSetDevice(h->device), "hipSetDevice");
@@ -865,6 +880,7 @@ int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size
   if (!h || !out || slots <= 0 || wg_per_slot <= 0 || max_points == 0 || max_points >= (1u << 31))
     return set_err(h, NDT_ERR_INVALID_ARGUMENT, "invalid server shape");
   *out = nullptr;
+  if (int rc = check_no_server(h)) return rc;
   if (!h->grid.ready) return set_err(h, NDT_ERR_NO_TARGET, "No target point cloud set");
   if ((size_t)slots * wg_per_slot > (size_t)std::max(1, h->coop_capacity))
     return set_err(h, NDT_ERR_CAPACITY, "slots x workgroups exceed the co-resident workgroups");
@@ -902,6 +918,7 @@ int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size
       hipSuccess)
     return fail(e, "server launch");
   s->running = true;
+  h->server = s;
   *out = s;
   return NDT_OK;
 }
@@ -967,6 +984,7 @@ int ndt_hip_server_stop(ndt_server* s) {
   if (s->mb) (void)hipHostFree(s->mb);
   if (s->ctl) (void)hipHostFree(s->ctl);
   if (s->host_points) (void)hipHostFree(s->host_points);
+  if (h && h->server == s) h->server = nullptr;
   delete s;
   return rc;
 }

@@ -446,8 +446,8 @@ class AlignServer:
     finished ``(ticket, AlignResult)`` pairs and admits queued requests.  A
     result equals ``align`` of the same scan and pose with
     ``workgroups_per_alignment = workgroups_per_slot``.  Use as a context
-    manager (``stop`` runs on exit); while it runs the matcher's other calls
-    wait for the device.
+    manager (``stop`` runs on exit).  While it runs, the matcher's other
+    calls raise NdtError: the server holds every co-resident workgroup.
     """
 
     def __init__(self, matcher: "NdtScanMatcher", slots: int, workgroups_per_slot: int, max_points: int):

@@ -236,8 +236,10 @@ int ndt_hip_laplace_covariance_6x6(const double hess[36], double cov[36]);
  * queued requests to free slots; stop lets running requests finish, drops
  * queued ones and ends the launch.  Results equal an align_scans /
  * ndt_hip_align of the same scan with workgroups_per_alignment = wg_per_slot.
- * While a server runs it occupies every slot's workgroups: other calls on
- * the handle wait for the GPU.  A slot left idle for about 10 s ends the
+ * While a server runs it holds the device's co-resident workgroups, so the
+ * handle's other calls (and a second start) return NDT_ERR_INVALID_ARGUMENT
+ * until stop; destroying the handle stops it.  A slot left idle for about
+ * 10 s ends the
  * launch; poll then reports NDT_ERR_HIP and the server must be restarted. */
 typedef struct ndt_server ndt_server;
 int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size_t max_points, ndt_server** out);

@@ -86,3 +86,32 @@ def test_server_errors(small):
         t = srv.submit(small.scan[:100], small.guess)
         got = _drain(srv, {}, 1, time.time() + 30)
         assert t in got
+
+
+def test_handle_refuses_other_work_while_serving(small):
+    """A running server holds the device's co-resident workgroups: the
+    handle's other calls and a second server are refused with an error (not
+    left waiting), and everything works again after stop."""
+    from cuda_ndt_matcher_amd import NdtError, NdtScanMatcher, NdtScanMatcherConfig
+
+    _print_case("server guard")
+    m = NdtScanMatcher(NdtScanMatcherConfig(workgroups_per_alignment=W))
+    m.set_target(small.map_points)
+    want = m.align(small.scan, small.guess)
+    srv = m.server(slots=2, workgroups_per_slot=W, max_points=small.scan.shape[0])
+    try:
+        with pytest.raises(NdtError):
+            m.align(small.scan, small.guess)
+        with pytest.raises(NdtError):
+            m.evaluate_nvtl(small.scan, small.guess)
+        with pytest.raises(NdtError):
+            m.set_target(small.map_points)
+        with pytest.raises(NdtError):
+            m.server(slots=1, workgroups_per_slot=W, max_points=100)
+        t = srv.submit(small.scan, small.guess)
+        got = _drain(srv, {}, 1, time.time() + 60)
+        assert t in got
+    finally:
+        srv.stop()
+    r = m.align(small.scan, small.guess)
+    assert np.array_equal(r.pose_vector, want.pose_vector) and got[t].iterations == want.iterations
