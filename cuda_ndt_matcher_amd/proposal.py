@@ -114,14 +114,18 @@ class ScanStore:
         self._r2 = (math.hypot(shape[0] / 2 * cell, shape[1] / 2 * cell) + cell) ** 2
         self._z = (zmin - zcell, zmin + (shape[2] + 1) * zcell)
 
+    def crop(self, scan) -> np.ndarray:
+        """The points of `scan` that some rotation about z can bin (host)."""
+        a = np.asarray(scan, np.float32).reshape(-1, 3)
+        keep = ((a[:, 0].astype(np.float64) ** 2 + a[:, 1].astype(np.float64) ** 2 <= self._r2)
+                & (a[:, 2] >= self._z[0]) & (a[:, 2] < self._z[1]))
+        return np.ascontiguousarray(a[keep])
+
     def add(self, scans: Sequence[np.ndarray]) -> None:
         import torch
 
         for s in scans:
-            a = np.asarray(s, np.float32).reshape(-1, 3)
-            keep = ((a[:, 0].astype(np.float64) ** 2 + a[:, 1].astype(np.float64) ** 2 <= self._r2)
-                    & (a[:, 2] >= self._z[0]) & (a[:, 2] < self._z[1]))
-            a = np.ascontiguousarray(a[keep])
+            a = self.crop(s)
             self._xyz.append(torch.from_numpy(a).to("cuda:0"))
             self._counts.append(a.shape[0])
 
