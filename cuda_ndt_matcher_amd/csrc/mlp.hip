@@ -45,7 +45,7 @@ constexpr int kTile = 32;      // MFMA output tile
 constexpr int kKStep = 16;     // K per MFMA
 constexpr int kWavesPerWg = 4; // four 32x32 tiles along N per workgroup
 
-enum Epilogue { kStore = 0, kBias = 1 };
+enum Epilogue { kStore = 0, kBias = 1, kPartial = 2 };
 
 // C[M x N] (f32) = A[M x K] B[N x K]^T (+ bias[N]).  M, N multiples of 32,
 // K a multiple of 16.  One wave per 32x32 tile; grid (N / 128, M / 32).
@@ -108,6 +108,93 @@ __global__ __launch_bounds__(64 * kWavesPerWg) void gemm_nt_splitk(const uint16_
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       out[(size_t)(tm + 8 * i + 4 * h + j) * N + tn + r] = acc0[4 * i + j] + acc1[4 * i + j];
+}
+
+// Macro-tiled form for the large products: a workgroup computes a 128x128
+// tile of C (each wave a 64x64 quadrant: 2x2 MFMA tiles), K in steps of 32
+// staged through double-buffered LDS with the next step's global loads in
+// flight during the current step's products.  Operand traffic per product is
+// a quarter of one 32x32 tile per wave.  grid (S, tiles): blockIdx.x is the K
+// slice [x * kchunk, min(K, (x + 1) * kchunk)), blockIdx.y the tile (row-major
+// over N / 128 columns); with S a multiple of 8 the slice index picks the XCD,
+// so workgroups that read the same K range share one L2.  M, N multiples of
+// 128; K and kchunk multiples of 32.  epi as for gemm_nt (kPartial: tile to
+// C + x * M * N).
+constexpr int kMacro = 128;
+constexpr int kKStage = 32;
+constexpr int kLdsRow = kKStage + 8;  // bf16 per LDS row (80 bytes: staggers the banks)
+__global__ __launch_bounds__(256) void gemm_nt_tiled(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
+                                                     float* __restrict__ C, const float* __restrict__ bias, int M,
+                                                     int N, int K, int kchunk, int epi) {
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][kMacro * kLdsRow];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][kMacro * kLdsRow];
+  const int tiles_n = N / kMacro;
+  const int tm = (int)(blockIdx.y / tiles_n) * kMacro, tn = (int)(blockIdx.y % tiles_n) * kMacro;
+  const int k0 = blockIdx.x * kchunk, k1 = min(K, k0 + kchunk);
+  const int steps = (k1 - k0) / kKStage;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int r = lane & 31, h = lane >> 5;
+  // copy assignment: 512 chunks of 8 bf16 per operand and stage; thread t
+  // takes chunks t and t + 256: row c / 4, k offset (c % 4) * 8
+  const int row0 = t >> 2, kseg = (t & 3) * 8;
+  const uint16_t* ga0 = A + (size_t)(tm + row0) * K + k0 + kseg;
+  const uint16_t* ga1 = A + (size_t)(tm + row0 + 64) * K + k0 + kseg;
+  const uint16_t* gb0 = B + (size_t)(tn + row0) * K + k0 + kseg;
+  const uint16_t* gb1 = B + (size_t)(tn + row0 + 64) * K + k0 + kseg;
+  const int l0 = row0 * kLdsRow + kseg, l1 = (row0 + 64) * kLdsRow + kseg;
+  bf16x8 pa0, pa1, pb0, pb1;
+  auto fetch = [&](int s) {
+    const int o = s * kKStage;
+    pa0 = *reinterpret_cast<const bf16x8*>(ga0 + o);
+    pa1 = *reinterpret_cast<const bf16x8*>(ga1 + o);
+    pb0 = *reinterpret_cast<const bf16x8*>(gb0 + o);
+    pb1 = *reinterpret_cast<const bf16x8*>(gb1 + o);
+  };
+  auto stash = [&](int buf) {
+    *reinterpret_cast<bf16x8*>(&sA[buf][l0]) = pa0;
+    *reinterpret_cast<bf16x8*>(&sA[buf][l1]) = pa1;
+    *reinterpret_cast<bf16x8*>(&sB[buf][l0]) = pb0;
+    *reinterpret_cast<bf16x8*>(&sB[buf][l1]) = pb1;
+  };
+  floatx16 acc[2][2] = {};
+  if (steps > 0) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
+  for (int s = 0; s < steps; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < steps) fetch(s + 1);
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh) {
+      bf16x8 fa[2], fb[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        fa[i] = *reinterpret_cast<const bf16x8*>(&sA[buf][(64 * wm + 32 * i + r) * kLdsRow + 16 * kh + 8 * h]);
+        fb[i] = *reinterpret_cast<const bf16x8*>(&sB[buf][(64 * wn + 32 * i + r) * kLdsRow + 16 * kh + 8 * h]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+    if (s + 1 < steps) stash(buf ^ 1);
+    __syncthreads();
+  }
+  float* out = C + (epi == kPartial ? (size_t)blockIdx.x * M * N : 0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = tn + 64 * wn + 32 * j + r;
+      const float bb = (epi == kBias) ? bias[col] : 0.0f;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          out[(size_t)(tm + 64 * wm + 32 * i + 8 * a + 4 * h + b) * N + col] = acc[i][j][4 * a + b] + bb;
+    }
 }
 
 // C[i] = sum over s = 0 .. S-1 of P[s * MN + i] (in that order) + bias[i % N].
@@ -337,8 +424,28 @@ void splitk_plan(int M, int N, int K, int* S, int* kchunk) {
   *S = (K + kc - 1) / kc;
 }
 
+// Macro-tile plan: shapes that fit 128x128 tiles, with K split into slices
+// (a multiple of 8 of them when there are at least 8) until about 256
+// workgroups run; false when even then fewer than 128 would (the small
+// products stay on gemm_nt).
+bool tiled_plan(int M, int N, int K, int* S, int* kchunk) {
+  if (M % kMacro || N % kMacro || K % kKStage) return false;
+  const int tiles = (M / kMacro) * (N / kMacro);
+  int s = 1;
+  if (tiles < 256) s = std::min((256 + tiles - 1) / tiles, std::max(1, K / 512));
+  if (s >= 8) s = s / 8 * 8;
+  int kc = (K + s - 1) / s;
+  kc = (kc + kKStage - 1) / kKStage * kKStage;
+  s = (K + kc - 1) / kc;
+  if (tiles * s < 128) return false;
+  *S = s;
+  *kchunk = kc;
+  return true;
+}
+
 size_t splitk_floats(int M, int N, int K) {
   int S, kc;
+  if (tiled_plan(M, N, K, &S, &kc)) return S > 1 ? (size_t)S * M * N : 0;
   splitk_plan(M, N, K, &S, &kc);
   return S > 1 ? (size_t)S * M * N : 0;
 }
@@ -385,6 +492,19 @@ int ensure_batch(ndt_mlp* m, int B) {
 void launch_gemm(ndt_mlp* m, const uint16_t* A, const uint16_t* B, float* C, const float* bias, int M, int N, int K) {
   const dim3 grid((unsigned)((N + kTile * kWavesPerWg - 1) / (kTile * kWavesPerWg)), (unsigned)(M / kTile));
   int S, kc;
+  if (tiled_plan(M, N, K, &S, &kc) && (S == 1 || (size_t)S * M * N <= m->splitk_cap)) {
+    const dim3 tg((unsigned)S, (unsigned)((M / kMacro) * (N / kMacro)));
+    if (S == 1) {
+      hipLaunchKernelGGL(gemm_nt_tiled, tg, dim3(256), 0, m->st, A, B, C, bias, M, N, K, kc,
+                         bias ? (int)kBias : (int)kStore);
+    } else {
+      hipLaunchKernelGGL(gemm_nt_tiled, tg, dim3(256), 0, m->st, A, B, m->splitk, nullptr, M, N, K, kc,
+                         (int)kPartial);
+      const size_t mn = (size_t)M * N;
+      hipLaunchKernelGGL(splitk_reduce, dim3(grid1(mn)), dim3(256), 0, m->st, m->splitk, C, bias, mn, N, S);
+    }
+    return;
+  }
   splitk_plan(M, N, K, &S, &kc);
   if (S > 1 && (size_t)S * M * N <= m->splitk_cap && K % (2 * kKStep) == 0) {
     hipLaunchKernelGGL(gemm_nt_splitk, dim3(grid.x, grid.y, (unsigned)S), dim3(64 * kWavesPerWg), 0, m->st, A, B,

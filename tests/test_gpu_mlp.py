@@ -12,11 +12,13 @@ def _print_case(msg):
     print(f"[case] {msg}", file=sys.stderr, flush=True)
 
 
-@pytest.mark.parametrize("sizes,batch", [((64, 64, 32, 64), 32), ((32768, 512, 512, 512, 64), 32)])
+@pytest.mark.parametrize("sizes,batch", [((64, 64, 32, 64), 32), ((32768, 512, 512, 512, 64), 32),
+                                         ((32768, 512, 512, 512, 64), 256)])
 def test_mlp_forward_backward_match_reference(sizes, batch):
     """MFMA forward/backward equal the numpy restatement that rounds the
     same operands to bf16 (only the accumulation order differs), and stay
-    within bf16 error of plain FP32."""
+    within bf16 error of plain FP32.  Batch 256 runs the layer-0 products on
+    the macro-tiled kernel (forward split over K, weight gradient whole)."""
     from cuda_ndt_matcher_amd.proposal import ProposalNet
     from oracle import mlp_ref as R
 
