@@ -36,8 +36,9 @@ def _same(a, b):
     for x, y in zip(a, b):
         assert np.array_equal(x.pose_vector, y.pose_vector)
         assert x.iterations == y.iterations and x.status == y.status
-        assert x.score == y.score and x.nvtl == y.nvtl and x.transform_probability == y.transform_probability
-        assert np.array_equal(x.hessian, y.hessian)
+        for a, b in ((x.score, y.score), (x.nvtl, y.nvtl), (x.transform_probability, y.transform_probability)):
+            assert np.array_equal(a, b, equal_nan=True)
+        assert np.array_equal(x.hessian, y.hessian, equal_nan=True)
         assert x.num_correspondences == y.num_correspondences
 
 
@@ -124,3 +125,29 @@ def test_phase_profile_on_the_first_call(small):
     r = m.align(small.scan, small.guess)
     ph = m.phase_profile()
     assert ph["iterations"] == r.iterations and ph["total_us"] > 0 and ph["derivative_us"] > 0
+
+
+def test_batch_beyond_coresident_capacity(small):
+    """More alignments than workgroups that fit on the GPU at once (one
+    workgroup each, no cross-workgroup barrier): every result equals the
+    same alignment run alone with one workgroup."""
+    _print_case("batch of 600")
+    m = _matcher(small.map_points, workgroups_per_alignment=1)
+    poses = _poses(small, 500, 600)
+    res = m.align_batch(small.scan, poses)
+    assert m.timing()["align_workgroups"] == 600
+    for k in (0, 257, 599):
+        _same([res[k]], [m.align(small.scan, poses[k])])
+
+
+def test_tiny_scans(small):
+    """One- and few-point scans align (or report no correspondences) without
+    error, and the multi-scan path agrees with single alignments."""
+    _print_case("tiny scans")
+    m = _matcher(small.map_points, workgroups_per_alignment=2)
+    scans = [(small.scan[:1], small.guess), (small.scan[:3], small.guess), (small.scan[:17], small.guess)]
+    many = m.align_parallel_scans(scans)
+    for (s, p), r in zip(scans, many):
+        alone = m.align(s, p)
+        _same([r], [alone])
+        assert r.status in ("Converged", "MaxIterations", "NoCorrespondences", "SingularHessian")
