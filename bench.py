@@ -237,7 +237,7 @@ def serve_bench(m, wl, seed, n_requests=400):
     return out
 
 
-def learned_bench(seed, n_train=196608, n_test=64, epochs=50, half=60.0):
+def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0):
     """Learned proposal on the 200k-point map: train the MLP on n_train
     seeded scans over [-half, half]^2 (labels: NDT poses converged from the
     true poses; every batch re-rotates the scans to fresh headings), then
@@ -353,8 +353,8 @@ def main():
                     help="also run the FP32/FP64 sweep over 5k-60k-point scans")
     ap.add_argument("--learned", action="store_true",
                     help="also train the proposal MLP and compare it with the plain particle search")
-    ap.add_argument("--learned-train", type=int, default=196608, help="training scans for --learned")
-    ap.add_argument("--learned-epochs", type=int, default=50, help="training epochs for --learned")
+    ap.add_argument("--learned-train", type=int, default=262144, help="training scans for --learned")
+    ap.add_argument("--learned-epochs", type=int, default=60, help="training epochs for --learned")
     ap.add_argument("--serve", action="store_true",
                     help="also serve a Poisson stream of 1k-40k-point requests (continuous batching)")
     ap.add_argument("--batching", action="store_true",
