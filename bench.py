@@ -237,7 +237,7 @@ def serve_bench(m, wl, seed, n_requests=400):
     return out
 
 
-def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0):
+def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cpu=True):
     """Learned proposal on the 200k-point map: train the MLP on n_train
     seeded scans over [-half, half]^2 (labels: NDT poses converged from the
     true poses; every batch re-rotates the scans to fresh headings), then
@@ -301,6 +301,18 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0):
         learn_ok += close(r.pose_vector, ref[k])
         agree += close(r.pose_vector, rp.pose_vector)
         fallbacks += fb
+    cpu = None
+    if with_cpu:
+        cases = []
+        for k, smp in enumerate(test[:16]):
+            r = np.random.default_rng(seed + k)
+            p0 = smp.true_pose
+            cases.append((smp.scan, [p0 + np.array([r.uniform(-2, 2), r.uniform(-2, 2), 0.0, r.normal(0, math.radians(1)),
+                                                    r.normal(0, math.radians(1)), r.uniform(-math.pi, math.pi)])
+                                     for _ in range(200)]))
+        cpu = cpu_particle_baseline(wl.map_points, cases)
+        print(f"[bench] plain search CPU: {cpu['value']:.3f} solves/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
     out = {"map": "200k-point procedural map", "train_scans": n_train, "train_points": train_points,
            "test_scans": n_test, "epochs": epochs, "region_m": 2 * half, "data_s": t_data, "train_s": t_train,
            "loss_first_last": [losses[0], losses[-1]],
@@ -311,7 +323,7 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0):
            "speedup": plain_t / learn_t,
            "best_hypothesis_err_m_p50_p90": [float(np.percentile(best_xy, 50)), float(np.percentile(best_xy, 90))],
            "best_hypothesis_err_deg_p50_p90": [float(np.percentile(best_yaw, 50)), float(np.percentile(best_yaw, 90))],
-           "within_1m_15deg": near}
+           "within_1m_15deg": near, "cpu_baseline_plain_search": cpu}
     print(f"[bench] learned proposal: loss {losses[0]:.3f} -> {losses[-1]:.3f} ({t_data:.1f} s data, {t_train:.1f} s "
           f"training); plain {plain_ok}/{n_test} at {n_test / plain_t:.1f} solves/s, learned {learn_ok}/{n_test} at "
           f"{n_test / learn_t:.1f} solves/s ({fallbacks} fallbacks, {direct} right without one, {agree} agree with plain); best hypothesis p50 "
@@ -336,6 +348,38 @@ def cpu_baseline(wl, budget_s=12.0):
     return {"value": k / el, "unit": "aligns/s", "cores": 1, "kind": "port",
             "sample": f"{k} single-core aligns of the same {wl.scan.shape[0]}-pt scan against the same 5M-pt map "
                       f"(C++ FP64 restatement of the reference CPU path, -O2), {el:.1f} s"}
+
+
+def cpu_particle_baseline(map_points, cases, budget_s=12.0, threads=None):
+    """The plain particle search on the host cores: the oracle (C++ FP64
+    restatement of the reference CPU path) aligns the particles of each case
+    (scan, list of starting poses) on a thread pool, as the reference's
+    align_batch does with rayon over particles; the best NVTL is picked.
+    Runs cases until the budget is spent (at least one)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    g = O.Grid(map_points, 2.0)
+    p = O.params()
+
+    def one(scan, pose):
+        r = g.align(scan, pose, p)
+        return g.nvtl(scan, r.pose, p)
+
+    t0 = time.perf_counter()
+    done = 0
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        for scan, poses in cases:
+            list(ex.map(lambda q: one(scan, q), poses))
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "init-pose solves/s", "cores": threads, "kind": "port",
+            "sample": f"{done} searches of {len(cases[0][1])} particles, one alignment per particle and NVTL, "
+                      f"on {threads} host threads (ctypes calls release the interpreter lock), {el:.1f} s"}
 
 
 def main():
@@ -451,6 +495,10 @@ def main():
                               "particles_within_2cm": ok, "workgroups_per_particle": m.timing()["align_workgroups"]}
         print(f"[bench] init pose: {args.particles} particles in {1e3 * bt:.2f} ms, best err {100 * err_t:.2f} cm "
               f"{err_r:.3f} deg, {ok} particles within 2 cm", file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            extra["init_pose"]["cpu_baseline"] = cpu_particle_baseline(wl.map_points, [(wl.scan, parts)])
+            print(f"[bench] init pose CPU: {extra['init_pose']['cpu_baseline']['value']:.3f} solves/s on "
+                  f"{extra['init_pose']['cpu_baseline']['cores']} threads", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,
@@ -480,7 +528,8 @@ def main():
     if args.serve:
         extra["serve"] = serve_bench(m, wl, args.seed)
     if args.learned:
-        extra["learned_proposal"] = learned_bench(args.seed, n_train=args.learned_train, epochs=args.learned_epochs)
+        extra["learned_proposal"] = learned_bench(args.seed, n_train=args.learned_train, epochs=args.learned_epochs,
+                                                   with_cpu=not args.no_cpu)
     if args.batching:
         extra["batching"] = batching_bench(wl, args.seed)
     if args.precision_sweep:
