@@ -68,6 +68,8 @@ class ProposalNet:
         self._check(self._lib.ndt_hip_mlp_set_params(self._h, p))
 
     def forward_device(self, d_x: int, batch: int) -> np.ndarray:
+        """Forward of bf16 device inputs; the caller makes sure d_x is written
+        (the library's stream is not ordered after other streams)."""
         out = np.zeros((batch, self.sizes[-1]), np.float32)
         self._check(self._lib.ndt_hip_mlp_forward_device(self._h, d_x, batch, out))
         return out
@@ -283,6 +285,9 @@ def propose(net: ProposalNet, frame: Frame, bevs, k: int) -> np.ndarray:
     b = bevs.shape[0]
     pad = (-b) % 32
     x = bevs if pad == 0 else torch.cat([bevs, bevs[:1].expand(pad, -1)]).contiguous()
+    # The network runs on the library's own stream, which is not ordered after
+    # torch's: the padded batch must be complete before the launch.
+    torch.cuda.synchronize()
     out = net.forward_device(x.data_ptr(), x.shape[0])[:b]
     hyp = frame.decode(out)
     order = np.argsort(-hyp[..., 3], axis=1)[:, :k]

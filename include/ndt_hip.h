@@ -12,7 +12,10 @@
  *    otherwise; ndt_hip_last_error(h) gives a message for the last failure.
  *  - Points are AoS float triples [[x,y,z],...] as in the reference's
  *    `&[[f32; 3]]`.  Host arrays are borrowed for the duration of the call.
- *    `*_device` variants take device pointers (HBM-resident inputs).
+ *    `*_device` variants take device pointers (HBM-resident inputs); the
+ *    library works on its own non-blocking stream, so such inputs must be
+ *    completely written (their producer's stream synchronised) before the
+ *    call.
  *  - A pose is the internal 6-vector [tx,ty,tz,roll,pitch,yaw] (f64) that the
  *    reference produces with isometry_to_pose_vector
  *    (src/ndt_cuda/src/optimization/types.rs:164-174, nalgebra euler_angles)
@@ -266,7 +269,10 @@ const char* ndt_hip_mlp_last_error(const ndt_mlp* m);
 size_t ndt_hip_mlp_num_params(const ndt_mlp* m);
 int ndt_hip_mlp_get_params(ndt_mlp* m, float* params);
 int ndt_hip_mlp_set_params(ndt_mlp* m, const float* params);
-/* Forward of bf16 inputs in device memory (batch x sizes[0]); out: host, batch x sizes[last]. */
+/* Forward of bf16 inputs in device memory (batch x sizes[0]); out: host, batch x sizes[last].
+ * The network runs on its own stream, which is not ordered after other
+ * streams: d_x must be completely written before the call (as for the
+ * training step's inputs). */
 int ndt_hip_mlp_forward_device(ndt_mlp* m, const uint16_t* d_x, size_t batch, float* out);
 /* Forward and backward of host inputs (rounded to bf16) with a given output
  * gradient dout; returns the outputs and the parameter gradients. */
