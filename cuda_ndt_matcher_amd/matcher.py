@@ -32,21 +32,36 @@ class NdtError(RuntimeError):
         self.code = code
 
 
-@dataclass
 class AlignResult:
-    """AlignResult (ndt.rs:220-248) plus the convergence status."""
+    """AlignResult (ndt.rs:220-248) plus the convergence status.  `pose` (an
+    Isometry3) is formed from `pose_vector` when first read."""
 
-    pose: Isometry3
-    pose_vector: np.ndarray
-    converged: bool
-    status: str
-    score: float
-    transform_probability: float
-    nvtl: float
-    iterations: int
-    hessian: np.ndarray
-    num_correspondences: int
-    oscillation_count: int
+    __slots__ = ("pose_vector", "converged", "status", "score", "transform_probability", "nvtl", "iterations",
+                 "hessian", "num_correspondences", "oscillation_count", "_pose")
+
+    def __init__(self, pose_vector, converged, status, score, transform_probability, nvtl, iterations, hessian,
+                 num_correspondences, oscillation_count, pose=None):
+        self.pose_vector = pose_vector
+        self.converged = converged
+        self.status = status
+        self.score = score
+        self.transform_probability = transform_probability
+        self.nvtl = nvtl
+        self.iterations = iterations
+        self.hessian = hessian
+        self.num_correspondences = num_correspondences
+        self.oscillation_count = oscillation_count
+        self._pose = pose
+
+    @property
+    def pose(self) -> Isometry3:
+        if self._pose is None:
+            self._pose = Isometry3.from_pose_vector(self.pose_vector)
+        return self._pose
+
+    def __repr__(self) -> str:
+        return (f"AlignResult(pose_vector={self.pose_vector!r}, status={self.status!r}, "
+                f"iterations={self.iterations}, score={self.score!r}, nvtl={self.nvtl!r})")
 
 
 @dataclass
@@ -187,14 +202,23 @@ def _pose_vec(pose) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(pose, dtype=np.float64).reshape(6))
 
 
+def _results(arr, m: int) -> List[AlignResult]:
+    """AlignResults of the first m entries of a ctypes ndt_result array, read
+    through one structured numpy view."""
+    a = np.ctypeslib.as_array(arr)[:m]
+    poses = np.array(a["pose"], dtype=np.float64)
+    hess = np.array(a["hessian"], dtype=np.float64).reshape(m, 6, 6)
+    status, conv = a["status"].tolist(), a["converged"].tolist()
+    score, tp, nvtl = a["score"].tolist(), a["transform_probability"].tolist(), a["nvtl"].tolist()
+    its, corr, osc = a["iterations"].tolist(), a["num_correspondences"].tolist(), a["oscillation_count"].tolist()
+    return [AlignResult(poses[i], bool(conv[i]), STATUS_NAMES.get(status[i], str(status[i])), score[i], tp[i], nvtl[i],
+                        its[i], hess[i], corr[i], osc[i]) for i in range(m)]
+
+
 def _result(r: _lib.NdtResult) -> AlignResult:
-    pv = np.array(r.pose[:])
-    return AlignResult(
-        pose=Isometry3.from_pose_vector(pv), pose_vector=pv, converged=bool(r.converged),
-        status=STATUS_NAMES.get(r.status, str(r.status)), score=r.score,
-        transform_probability=r.transform_probability, nvtl=r.nvtl, iterations=int(r.iterations),
-        hessian=np.array(r.hessian[:]).reshape(6, 6), num_correspondences=int(r.num_correspondences),
-        oscillation_count=int(r.oscillation_count))
+    arr = (_lib.NdtResult * 1)()
+    arr[0] = r
+    return _results(arr, 1)[0]
 
 
 class NdtScanMatcher:
@@ -331,7 +355,7 @@ class NdtScanMatcher:
         poses = np.ascontiguousarray(np.stack([_pose_vec(x) for x in initial_poses]).reshape(-1))
         res = (_lib.NdtResult * m)()
         self._check(self._lib.ndt_hip_align_batch(self._h, p, p.shape[0], poses, m, res))
-        return [_result(res[i]) for i in range(m)]
+        return _results(res, m)
 
     align_batch_gpu = align_batch
 
@@ -340,7 +364,7 @@ class NdtScanMatcher:
         poses = np.ascontiguousarray(np.stack([_pose_vec(x) for x in initial_poses]).reshape(-1))
         res = (_lib.NdtResult * m)()
         self._check(self._lib.ndt_hip_align_batch_device(self._h, C.c_void_p(ptr), n, poses, m, res))
-        return [_result(res[i]) for i in range(m)]
+        return _results(res, m)
 
     def align_parallel_scans(self, scans: Sequence[Tuple[object, object]]) -> List[AlignResult]:
         """align_parallel_scans (ndt.rs:1224-1320): uneven scans, one launch."""
@@ -353,7 +377,7 @@ class NdtScanMatcher:
         poses = np.ascontiguousarray(np.stack([_pose_vec(p) for _, p in scans]).reshape(-1))
         res = (_lib.NdtResult * m)()
         self._check(self._lib.ndt_hip_align_scans(self._h, ptrs, ns, poses, m, res))
-        return [_result(res[i]) for i in range(m)]
+        return _results(res, m)
 
     def server(self, slots: int, workgroups_per_slot: int, max_points: int) -> "AlignServer":
         """Continuous-batching server over this matcher's map (see AlignServer)."""
@@ -470,7 +494,7 @@ class AlignServer:
     def poll(self) -> List[Tuple[int, AlignResult]]:
         cnt = C.c_size_t()
         self._m._check(self._lib.ndt_hip_server_poll(self._s, self._res, self._tk, self._cap, C.byref(cnt)))
-        return [(int(self._tk[i]), _result(self._res[i])) for i in range(cnt.value)]
+        return list(zip([int(t) for t in self._tk[:cnt.value]], _results(self._res, cnt.value)))
 
     def stop(self) -> None:
         if self._s:
