@@ -563,8 +563,8 @@ void refresh_w_copy(ndt_mlp* m, int l) {
   hipLaunchKernelGGL(to_bf16, dim3(grid1((size_t)N * K)), dim3(256), 0, m->st, m->w[l], m->wt_bf[l], (size_t)N * K);
 }
 
-bool shape_ok(const ndt_mlp* m, int B) {
-  if (B <= 0 || B % kTile) return false;
+bool shape_ok(const ndt_mlp* m, size_t batch) {
+  if (batch == 0 || batch > (1u << 24) || batch % kTile) return false;
   for (int l = 0; l < m->layers; ++l)
     if (m->sizes[l] % kTile || m->sizes[l + 1] % kTile || m->sizes[l] < kKStep) return false;
   return true;
@@ -679,7 +679,7 @@ int ndt_hip_mlp_set_params(ndt_mlp* m, const float* params) {
 }
 
 int ndt_hip_mlp_forward_device(ndt_mlp* m, const uint16_t* d_x, size_t batch, float* out) {
-  if (!m || !d_x || !out || !shape_ok(m, (int)batch)) return NDT_ERR_INVALID_ARGUMENT;
+  if (!m || !d_x || !out || !shape_ok(m, batch)) return NDT_ERR_INVALID_ARGUMENT;
   const int B = (int)batch;
   int rc = ensure_batch(m, B);
   if (rc) return rc;
@@ -694,7 +694,7 @@ int ndt_hip_mlp_forward_device(ndt_mlp* m, const uint16_t* d_x, size_t batch, fl
 
 int ndt_hip_mlp_forward_backward(ndt_mlp* m, const float* x, size_t batch, const float* dout, float* out,
                                  float* grads) {
-  if (!m || !x || !dout || !out || !grads || !shape_ok(m, (int)batch)) return NDT_ERR_INVALID_ARGUMENT;
+  if (!m || !x || !dout || !out || !grads || !shape_ok(m, batch)) return NDT_ERR_INVALID_ARGUMENT;
   const int B = (int)batch;
   int rc = ensure_batch(m, B);
   if (rc) return rc;
@@ -719,7 +719,7 @@ int ndt_hip_mlp_forward_backward(ndt_mlp* m, const float* x, size_t batch, const
 
 int ndt_hip_mlp_train_step_device(ndt_mlp* m, const uint16_t* d_x, const float* labels, size_t batch,
                                   float learning_rate, float* mean_loss) {
-  if (!m || !d_x || !labels || !mean_loss || !shape_ok(m, (int)batch) || m->sizes.back() % 4)
+  if (!m || !d_x || !labels || !mean_loss || !shape_ok(m, batch) || m->sizes.back() % 4)
     return NDT_ERR_INVALID_ARGUMENT;
   const int B = (int)batch, H = m->sizes.back() / 4;
   int rc = ensure_batch(m, B);
