@@ -449,12 +449,13 @@ void ndt_hip_destroy(ndt_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   ndt::DeviceGrid& G = h->grid;
   for (DevBuf* b : {&G.owned_points, &G.sort_keys_a, &G.sort_keys_b, &G.sort_vals_a, &G.sort_vals_b, &G.hist,
-                    &G.status, &G.tile_heads, &G.seg_start, &G.scalars, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
+                    &G.status, &G.tile_heads, &G.seg_start, &G.scalars, &G.block_part, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
                     &G.valid, &G.table, &G.bitmap, &G.cell_nbr, &G.cell_ent, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
                     &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
   h->gcache.reset();
   if (G.host_scalars) (void)hipHostFree(G.host_scalars);
+  if (G.host_part) (void)hipHostFree(G.host_part);
   if (h->host_results) (void)hipHostFree(h->host_results);
   if (h->host_jobs) (void)hipHostFree(h->host_jobs);
   for (hipEvent_t ev : {h->ev0, h->ev1, G.ev_build0, G.ev_build1, G.ev_sort0, G.ev_sort1})

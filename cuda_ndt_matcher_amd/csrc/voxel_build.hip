@@ -40,36 +40,48 @@ __global__ void fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Min/max of the per-point voxel keys: wave shuffles, then LDS across the
-// workgroup, then one atomic per value per workgroup.
-__device__ inline void block_minmax_commit(int32_t mn[3], int32_t mx[3], int32_t* bounds) {
+// Per-workgroup min/max (and count) to part[8 * block .. 8 * block + 6]
+// (min x, y, z, max x, y, z, count), no atomics: the host reads the parts back
+// with the copy it waits for anyway and reduces them.
+constexpr int kMaxBoundBlocks = 1024;
+__device__ inline void block_minmax_store(int32_t mn[3], int32_t mx[3], uint32_t cnt, int32_t* part) {
   __shared__ int32_t s_mn[kBuildBlock / kWave][3], s_mx[kBuildBlock / kWave][3];
-  for (int a = 0; a < 3; ++a) {
+  __shared__ uint32_t s_c[kBuildBlock / kWave];
+  for (int a = 0; a < 3; ++a)
     for (int off = 32; off >= 1; off >>= 1) {
       mn[a] = min(mn[a], __shfl_xor(mn[a], off, kWave));
       mx[a] = max(mx[a], __shfl_xor(mx[a], off, kWave));
     }
-  }
+  for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, kWave);
   const int wave = threadIdx.x / kWave;
-  if ((threadIdx.x & (kWave - 1)) == 0)
+  if ((threadIdx.x & (kWave - 1)) == 0) {
     for (int a = 0; a < 3; ++a) {
       s_mn[wave][a] = mn[a];
       s_mx[wave][a] = mx[a];
     }
+    s_c[wave] = cnt;
+  }
   __syncthreads();
-  if (threadIdx.x < 3) {
-    int32_t a = INT32_MAX, b = INT32_MIN;
-    for (int w = 0; w < kBuildBlock / kWave; ++w) {
-      a = min(a, s_mn[w][threadIdx.x]);
-      b = max(b, s_mx[w][threadIdx.x]);
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBuildBlock / kWave; ++w) {
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = min(mn[a], s_mn[w][a]);
+        mx[a] = max(mx[a], s_mx[w][a]);
+      }
+      cnt += s_c[w];
     }
-    atomicMin(&bounds[threadIdx.x], a);
-    atomicMax(&bounds[3 + threadIdx.x], b);
+    int32_t* q = part + 8 * blockIdx.x;
+    for (int a = 0; a < 3; ++a) {
+      q[a] = mn[a];
+      q[3 + a] = mx[a];
+    }
+    q[6] = (int32_t)cnt;
+    q[7] = 0;
   }
 }
 
 __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restrict__ pts, size_t n, float res,
-                                                          int32_t* bounds /* minx,miny,minz,maxx,maxy,maxz */) {
+                                                          int32_t* part) {
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   const size_t S = (size_t)gridDim.x * blockDim.x;
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -97,7 +109,7 @@ __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restric
       mx[a] = max(mx[a], k);
     }
   }
-  block_minmax_commit(mn, mx, bounds);
+  block_minmax_store(mn, mx, 0u, part);
 }
 
 // Interleaves the bits of (rx, ry, rz) with widths (bx, by, bz): bit i of each
@@ -537,20 +549,22 @@ __global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t
 // Bounding box of the valid voxel coordinates (for the lookup's fast reject).
 __global__ __launch_bounds__(kBuildBlock) void valid_bounds(const int32_t* __restrict__ coords,
                                                             const uint32_t* __restrict__ valid, uint32_t segments,
-                                                            int32_t* bounds, uint32_t* nvalid) {
+                                                            int32_t* part) {
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   uint32_t c = 0;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < segments; s += gridDim.x * blockDim.x) {
-    if (!valid[s]) continue;
+    const uint32_t v = valid[s];
+    const int32_t k0 = coords[3 * s], k1 = coords[3 * s + 1], k2 = coords[3 * s + 2];  // loaded with valid
+    if (!v) continue;
     ++c;
-    for (int a = 0; a < 3; ++a) {
-      mn[a] = min(mn[a], coords[3 * s + a]);
-      mx[a] = max(mx[a], coords[3 * s + a]);
-    }
+    mn[0] = min(mn[0], k0);
+    mn[1] = min(mn[1], k1);
+    mn[2] = min(mn[2], k2);
+    mx[0] = max(mx[0], k0);
+    mx[1] = max(mx[1], k1);
+    mx[2] = max(mx[2], k2);
   }
-  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, kWave);
-  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(nvalid, c);
-  block_minmax_commit(mn, mx, bounds);
+  block_minmax_store(mn, mx, c, part);
 }
 
 __global__ void init_bounds(int32_t* b, uint32_t* extra, int nextra) {
@@ -652,13 +666,38 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   int32_t* bounds = reinterpret_cast<int32_t*>(G.scalars.ptr) + 8;
   uint32_t* extra = reinterpret_cast<uint32_t*>(G.scalars.ptr);
   hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra, 8);
-  hipLaunchKernelGGL(key_bounds, dim3(grid_for(n, kBuildBlock, 1024)), dim3(kBuildBlock), 0, st, G.points, n, res,
-                     bounds);
-  if ((e = hipMemcpyAsync(G.host_scalars + 8, bounds, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+  if ((e = G.block_part.reserve((size_t)kMaxBoundBlocks * 8 * sizeof(int32_t))) != hipSuccess)
+    return fail(e, "alloc");
+  if (!G.host_part &&
+      (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_part), (size_t)kMaxBoundBlocks * 8 * sizeof(int32_t),
+                         hipHostMallocDefault)) != hipSuccess)
+    return fail(e, "pinned alloc");
+  int32_t* part = reinterpret_cast<int32_t*>(G.block_part.ptr);
+  // per-workgroup partials reduced on the host (n > 0, so every block has an entry)
+  auto reduce_parts = [&](int blocks, int32_t out[6], uint32_t* count) {
+    for (int a = 0; a < 3; ++a) {
+      out[a] = INT32_MAX;
+      out[3 + a] = INT32_MIN;
+    }
+    uint64_t c = 0;
+    for (int b = 0; b < blocks; ++b) {
+      const int32_t* q = G.host_part + 8 * b;
+      for (int a = 0; a < 3; ++a) {
+        out[a] = std::min(out[a], q[a]);
+        out[3 + a] = std::max(out[3 + a], q[3 + a]);
+      }
+      c += (uint32_t)q[6];
+    }
+    if (count) *count = (uint32_t)c;
+  };
+  const int kb_blocks = grid_for(n, kBuildBlock, kMaxBoundBlocks);
+  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, part);
+  if ((e = hipMemcpyAsync(G.host_part, part, (size_t)kb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
+      hipSuccess)
     return fail(e, "bounds copy");
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "bounds sync");
   int32_t kb[6];
-  std::memcpy(kb, G.host_scalars + 8, sizeof(kb));
+  reduce_parts(kb_blocks, kb, nullptr);
   KeyParams kp;
   kp.res = res;
   int total_bits = 0;
@@ -720,15 +759,16 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
                      reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
                      reinterpret_cast<const float4*>(G.vox.ptr), segs, reinterpret_cast<HashEntry*>(G.table.ptr),
                      cap - 1);
-  hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra + 1, 1);
-  hipLaunchKernelGGL(valid_bounds, dim3(grid_for(segs, kBuildBlock, 256)), dim3(kBuildBlock), 0, st,
+  const int vb_blocks = grid_for(segs, kBuildBlock, kMaxBoundBlocks);
+  hipLaunchKernelGGL(valid_bounds, dim3(vb_blocks), dim3(kBuildBlock), 0, st,
                      reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                     segs, bounds, extra + 1);
-  if ((e = hipMemcpyAsync(G.host_scalars, extra, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+                     segs, part);
+  if ((e = hipMemcpyAsync(G.host_part, part, (size_t)vb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
+      hipSuccess)
     return fail(e, "voxel count copy");
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
-  G.num_valid = G.host_scalars[1];
-  const int32_t* vb = reinterpret_cast<const int32_t*>(G.host_scalars + 8);
+  int32_t vb[6];
+  reduce_parts(vb_blocks, vb, &G.num_valid);
   GridView& v = G.view;
   v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
   v.axis = reinterpret_cast<const float*>(G.axis.ptr);
