@@ -243,12 +243,13 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     true poses; every batch re-rotates the scans to fresh headings), then
     compare on n_test new scans the proposal search (16 hypotheses refined in
     one launch, 200-particle fallback below NVTL 2.3) with the plain
-    200-particle search.  Success: <= 2 cm / 0.1 deg of the alignment
+    200-particle search and the tree-structured Parzen estimator search
+    (200 particles, 100 startup).  Success: <= 2 cm / 0.1 deg of the alignment
     started at the true pose; agreement: the same bar against the plain
     search's answer."""
     import torch
 
-    from cuda_ndt_matcher_amd import NdtScanMatcher, proposal, synth
+    from cuda_ndt_matcher_amd import NdtScanMatcher, estimate_initial_pose, proposal, synth
 
     wl = synth.make_workload("small", seed)
     m = NdtScanMatcher.builder().resolution(2.0).max_iterations(30).transformation_epsilon(0.01).build()
@@ -288,11 +289,23 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     best_yaw = dyaw[np.arange(n_test), best_k]
     near = int(np.sum((best_xy < 1.0) & (best_yaw < 15.0)))
     plain_ok, plain_t, learn_ok, learn_t, fallbacks, agree, direct = 0, 0.0, 0, 0.0, 0, 0, 0
+    # The reference's own search: 200 particles around the prior, 100 startup
+    # particles in one launch, then 100 proposed by the tree-structured Parzen
+    # estimator, one at a time (its order) or ten per launch.  The prior's
+    # spread matches the plain search's: sigma_xy = 2/sqrt(3) m (the deviation
+    # of U(+-2 m)), z at its 0.1 m floor, roll/pitch 1 deg; yaw is uniform.
+    tpe_cov = np.diag([4.0 / 3.0, 4.0 / 3.0, 0.0, math.radians(1) ** 2, math.radians(1) ** 2, 0.0]).ravel()
+    tpe = {1: [0, 0.0], 10: [0, 0.0]}
     for k, smp in enumerate(test):
         t = time.perf_counter()
         rp = proposal.plain_search(m, smp, 200, seed + k)
         plain_t += time.perf_counter() - t
         plain_ok += close(rp.pose_vector, ref[k])
+        for gb, acc in tpe.items():
+            t = time.perf_counter()
+            rt = estimate_initial_pose(m, smp.scan, smp.true_pose, tpe_cov, guided_batch=gb, seed=seed + k)
+            acc[1] += time.perf_counter() - t
+            acc[0] += close(rt.pose, ref[k])
         t = time.perf_counter()
         r, fb = proposal.learned_search(m, net, frame, smp, tb[k:k + 1], fallback=lambda s_, k=k: proposal.plain_search(
             m, s_, 200, seed + k))
@@ -320,13 +333,17 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
            "learned": {"success_rate": learn_ok / n_test, "solves_per_s": n_test / learn_t,
                        "fallback_rate": fallbacks / n_test, "agrees_with_plain": agree / n_test,
                        "success_without_fallback": direct / n_test},
+           "tpe": {f"guided_batch_{gb}": {"success_rate": acc[0] / n_test, "solves_per_s": n_test / acc[1]}
+                   for gb, acc in tpe.items()},
            "speedup": plain_t / learn_t,
            "best_hypothesis_err_m_p50_p90": [float(np.percentile(best_xy, 50)), float(np.percentile(best_xy, 90))],
            "best_hypothesis_err_deg_p50_p90": [float(np.percentile(best_yaw, 50)), float(np.percentile(best_yaw, 90))],
            "within_1m_15deg": near, "cpu_baseline_plain_search": cpu}
     print(f"[bench] learned proposal: loss {losses[0]:.3f} -> {losses[-1]:.3f} ({t_data:.1f} s data, {t_train:.1f} s "
           f"training); plain {plain_ok}/{n_test} at {n_test / plain_t:.1f} solves/s, learned {learn_ok}/{n_test} at "
-          f"{n_test / learn_t:.1f} solves/s ({fallbacks} fallbacks, {direct} right without one, {agree} agree with plain); best hypothesis p50 "
+          f"{n_test / learn_t:.1f} solves/s ({fallbacks} fallbacks, {direct} right without one, {agree} agree with plain); "
+          f"TPE {tpe[1][0]}/{n_test} at {n_test / tpe[1][1]:.1f} solves/s (one guided particle per launch), "
+          f"{tpe[10][0]}/{n_test} at {n_test / tpe[10][1]:.1f} (ten per launch); best hypothesis p50 "
           f"{out['best_hypothesis_err_m_p50_p90'][0]:.2f} m {out['best_hypothesis_err_deg_p50_p90'][0]:.1f} deg, "
           f"{near}/{n_test} within 1 m / 15 deg", file=sys.stderr, flush=True)
     return out
