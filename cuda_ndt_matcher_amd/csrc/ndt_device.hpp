@@ -580,11 +580,15 @@ __device__ inline void sym_eigen(const double* a_in, double* ev, double* v) {
 //    lane shuffles.  It stops early when wave 1 has found H negative
 //    definite; otherwise the step is formed in the eigenbasis with the
 //    reference's clamp.
-// V is kept as the next call's starting basis (A = V^T H V), which for the
-// slowly changing Hessians of successive Newton iterations is nearly
-// diagonal.  Any orthonormal V is a valid start.
+// The eigenvectors of the last completed decomposition are kept as the next
+// call's starting basis (A = V^T H V), which for the slowly changing
+// Hessians of successive Newton iterations is nearly diagonal.  Any
+// orthonormal V is a valid start.  A call that takes the Cholesky step keeps
+// the previous basis: its Jacobi may have stopped at a point that depends
+// on timing, and every workgroup of a slot must hold the same state.
 struct SolveScratch {
-  double V[36];    // eigenvectors (columns), row-major
+  double V[36];    // starting basis: eigenvectors (columns), row-major
+  double Vn[36];   // this call's eigenvectors
   double T[36];
   double L[36];
   double dc[6];    // Cholesky step
@@ -770,8 +774,7 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
       jacobi_round<3>(aij, vij, i, j, live, lane);
       jacobi_round<4>(aij, vij, i, j, live, lane);
     }
-    if (live) W.V[lane] = vij;
-    if (lane == 0) W.have_basis = 1;
+    if (live) W.Vn[lane] = vij;
     if (!fast_exit) {
       // eigenvalues A_kk (lane 7k), the reference's clamp, and the step in
       // the eigenbasis: d = sum_k (V_k . (-g)) / lam_k V_k
@@ -794,7 +797,7 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
       if (lane < 6) {
         double proj = 0.0;
 #pragma unroll
-        for (int a = 0; a < 6; ++a) proj += W.V[a * 6 + lane] * (-g[a]);
+        for (int a = 0; a < 6; ++a) proj += W.Vn[a * 6 + lane] * (-g[a]);
         double lk = lam[0];
 #pragma unroll
         for (int k = 1; k < 6; ++k)
@@ -807,7 +810,7 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
         if (lane < 6) {
           double di = 0.0;
 #pragma unroll
-          for (int k = 0; k < 6; ++k) di += W.V[lane * 6 + k] * W.T[k];
+          for (int k = 0; k < 6; ++k) di += W.Vn[lane * 6 + k] * W.T[k];
           d[lane] = di;
         }
         if (lane == 0) W.ok = 1;
@@ -816,13 +819,15 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
     }
   }
   __syncthreads();
-  // When the Cholesky step exists the Jacobi may have stopped at a point that
-  // depends on timing; its basis is dropped so that every workgroup of a
-  // slot keeps the same state.
-  if (threadIdx.x < 6 && W.fast) d[threadIdx.x] = W.dc[threadIdx.x];
-  if (threadIdx.x == 0 && W.fast) {
-    W.ok = 1;
-    W.have_basis = 0;
+  // W.fast depends on H alone, so every workgroup of a slot makes the same
+  // choice: the Cholesky step (previous basis kept) or the eigenbasis step
+  // (its eigenvectors become the next start).
+  if (W.fast) {
+    if (threadIdx.x < 6) d[threadIdx.x] = W.dc[threadIdx.x];
+    if (threadIdx.x == 0) W.ok = 1;
+  } else {
+    if (threadIdx.x < 36) W.V[threadIdx.x] = W.Vn[threadIdx.x];
+    if (threadIdx.x == 0) W.have_basis = 1;
   }
   __syncthreads();
 }

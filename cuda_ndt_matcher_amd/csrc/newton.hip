@@ -451,10 +451,11 @@ struct SlotState {
 
 // Slot-wide exchange state, identical in every workgroup of a slot.
 //
-// Arrivals are counted in two levels so that no counter takes more than
-// kArrivalGroup + a few atomics per reduction: each workgroup adds to the
-// counter of its group of kArrivalGroup ranks, and the lane that completes a
-// group adds 1 to the slot's top counter, which every workgroup polls.
+// Arrivals are counted in two levels: each lane that stores a partial value
+// adds to the counter of its workgroup's group of kArrivalGroup ranks (its
+// own release covers its own store), and the lane that completes a group
+// adds 1 to the slot's top counter, which every workgroup polls.  (One
+// arrival per workgroup instead of one per value measured the same.)
 // Counters sit in separate 128-byte lines.
 constexpr int kArrivalGroup = 16;
 constexpr int kCounterStride = 32;  // uint32 per counter line
