@@ -86,6 +86,26 @@ class NdtTuningEntry(C.Structure):
     ]
 
 
+class NdtFilterParams(C.Structure):
+    _fields_ = [
+        ("min_distance", C.c_float),
+        ("max_distance", C.c_float),
+        ("min_z", C.c_float),
+        ("max_z", C.c_float),
+        ("downsample_resolution", C.c_float),
+        ("downsample", C.c_int32),
+    ]
+
+
+class NdtFilterCounts(C.Structure):
+    _fields_ = [
+        ("kept", C.c_uint64),
+        ("removed_by_distance", C.c_uint64),
+        ("removed_by_z", C.c_uint64),
+        ("removed_by_downsampling", C.c_uint64),
+    ]
+
+
 class NdtPhaseProfile(C.Structure):
     _fields_ = [
         ("derivative_us", C.c_float),
@@ -117,7 +137,8 @@ EXPORTS = [
     "ndt_hip_server_start", "ndt_hip_server_submit", "ndt_hip_server_poll", "ndt_hip_server_stop",
     "ndt_hip_mlp_create", "ndt_hip_mlp_destroy", "ndt_hip_mlp_last_error", "ndt_hip_mlp_num_params",
     "ndt_hip_mlp_get_params", "ndt_hip_mlp_set_params", "ndt_hip_mlp_forward_device", "ndt_hip_mlp_forward_backward",
-    "ndt_hip_mlp_train_step_device", "ndt_hip_bev_device",
+    "ndt_hip_mlp_train_step_device", "ndt_hip_bev_device", "ndt_hip_filter_params_default", "ndt_hip_filter_points",
+    "ndt_hip_filter_points_device", "ndt_hip_voxel_downsample",
 ]
 
 _lib = None
@@ -180,6 +201,13 @@ def load():
     lib.ndt_hip_get_timing.argtypes = [vp, C.POINTER(NdtTiming)]
     lib.ndt_hip_get_tuning.argtypes = [vp, C.POINTER(NdtTuningEntry), sz, C.POINTER(sz)]
     lib.ndt_hip_get_phase_profile.argtypes = [vp, C.POINTER(NdtPhaseProfile)]
+    lib.ndt_hip_filter_params_default.argtypes = [C.POINTER(NdtFilterParams)]
+    lib.ndt_hip_filter_params_default.restype = None
+    lib.ndt_hip_filter_points.argtypes = [vp, f32p, sz, C.POINTER(NdtFilterParams), f32p, sz, C.POINTER(sz),
+                                          C.POINTER(NdtFilterCounts)]
+    lib.ndt_hip_filter_points_device.argtypes = [vp, vp, sz, C.POINTER(NdtFilterParams), vp, sz, C.POINTER(sz),
+                                                 C.POINTER(NdtFilterCounts)]
+    lib.ndt_hip_voxel_downsample.argtypes = [vp, f32p, sz, C.c_float, f32p, sz, C.POINTER(sz)]
     lib.ndt_hip_server_start.argtypes = [vp, C.c_int32, C.c_int32, sz, C.POINTER(vp)]
     lib.ndt_hip_server_submit.argtypes = [vp, f32p, sz, f64p, C.POINTER(C.c_uint64)]
     lib.ndt_hip_server_poll.argtypes = [vp, rp, C.POINTER(C.c_uint64), sz, C.POINTER(sz)]

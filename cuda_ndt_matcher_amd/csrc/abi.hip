@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -25,6 +26,8 @@ struct ndt_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   ndt::DeviceGrid grid;
+  ndt::DeviceGrid filt;  // scratch of the sensor point filter (never a map)
+  DevBuf filt_in, filt_out;
   std::string err;
   DevBuf scan, jobs, results, ws, poses, partials, out, lookup_q, lookup_c, lookup_i, phase;
   ndt_phase_profile phase_profile{};
@@ -454,8 +457,16 @@ void ndt_hip_destroy(ndt_handle* h) {
                     &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
   h->gcache.reset();
-  if (G.host_scalars) (void)hipHostFree(G.host_scalars);
-  if (G.host_part) (void)hipHostFree(G.host_part);
+  ndt::DeviceGrid& F = h->filt;
+  for (DevBuf* b : {&F.owned_points, &F.sort_keys_a, &F.sort_keys_b, &F.sort_vals_a, &F.sort_vals_b, &F.hist,
+                    &F.status, &F.tile_heads, &F.seg_start, &F.scalars, &F.block_part, &h->filt_in, &h->filt_out})
+    b->release();
+  for (ndt::DeviceGrid* g : {&G, &F}) {
+    if (g->host_scalars) (void)hipHostFree(g->host_scalars);
+    if (g->host_part) (void)hipHostFree(g->host_part);
+  }
+  for (hipEvent_t ev : {F.ev_sort0, F.ev_sort1})
+    if (ev) (void)hipEventDestroy(ev);
   if (h->host_results) (void)hipHostFree(h->host_results);
   if (h->host_jobs) (void)hipHostFree(h->host_jobs);
   for (hipEvent_t ev : {h->ev0, h->ev1, G.ev_build0, G.ev_build1, G.ev_sort0, G.ev_sort1})
@@ -727,6 +738,86 @@ int ndt_hip_evaluate_nvtl_batch(ndt_handle* h, const float* xyz, size_t n, const
   rc = upload_points(h, h->scan, xyz, n, &d);
   if (rc) return rc;
   return run_score(h, ndt::kScoreNvtlCell27, d, n, poses, m, out);
+}
+
+void ndt_hip_filter_params_default(ndt_filter_params* p) {
+  if (!p) return;
+  p->min_distance = 0.0f;  // filtering/mod.rs:22-31
+  p->max_distance = FLT_MAX;
+  p->min_z = -FLT_MAX;
+  p->max_z = FLT_MAX;
+  p->downsample_resolution = 0.0f;
+  p->downsample = 0;
+}
+
+namespace {
+
+// filter_points on the handle's filter scratch: d_in (n points) -> d_out.
+int run_filter(ndt_handle* h, const float* d_in, size_t n, const ndt_filter_params* p, float* d_out, size_t out_cap,
+               size_t* out_n, ndt_filter_counts* counts) {
+  ndt::DeviceGrid& F = h->filt;
+  if (!F.ev_sort0) NDT_HIP(hipEventCreate(&F.ev_sort0), "event");
+  if (!F.ev_sort1) NDT_HIP(hipEventCreate(&F.ev_sort1), "event");
+  F.points = d_in;
+  uint64_t c[4];
+  std::string err;
+  const int rc = ndt::filter_points(F, n, p->min_distance, p->max_distance, p->min_z, p->max_z, p->downsample,
+                                    p->downsample_resolution, d_out, out_cap, out_n, c, h->stream, &err);
+  if (counts) {
+    counts->kept = c[0];
+    counts->removed_by_distance = c[1];
+    counts->removed_by_z = c[2];
+    counts->removed_by_downsampling = c[3];
+  }
+  if (rc != 0) return set_err(h, rc, err);
+  return NDT_OK;
+}
+
+int check_filter_args(ndt_handle* h, const void* in, size_t n, const ndt_filter_params* p, const void* out,
+                      const size_t* out_n) {
+  if (!h) return NDT_ERR_INVALID_ARGUMENT;
+  if (int rc = check_no_server(h)) return rc;
+  if (!p || !out_n || (n > 0 && (!in || !out))) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  NDT_HIP(hipSetDevice(h->device), "hipSetDevice");
+  return NDT_OK;
+}
+
+}  // namespace
+
+int ndt_hip_filter_points(ndt_handle* h, const float* xyz, size_t n, const ndt_filter_params* p, float* out,
+                          size_t out_cap, size_t* out_n, ndt_filter_counts* counts) {
+  if (int rc = check_filter_args(h, xyz, n, p, out, out_n)) return rc;
+  if (n == 0) return run_filter(h, nullptr, 0, p, nullptr, 0, out_n, counts);
+  const float* d = nullptr;
+  if (int rc = upload_points(h, h->filt_in, xyz, n, &d)) return rc;
+  NDT_HIP(h->filt_out.reserve(n * 3 * sizeof(float)), "alloc points");
+  float* d_out = reinterpret_cast<float*>(h->filt_out.ptr);
+  if (int rc = run_filter(h, d, n, p, d_out, n, out_n, counts)) return rc;
+  if (out_cap < *out_n) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "output capacity below the result count");
+  if (*out_n) {
+    NDT_HIP(hipMemcpyAsync(out, d_out, *out_n * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream), "copy out");
+    NDT_HIP(hipStreamSynchronize(h->stream), "filter sync");
+  }
+  return NDT_OK;
+}
+
+int ndt_hip_filter_points_device(ndt_handle* h, const float* d_xyz, size_t n, const ndt_filter_params* p,
+                                 float* d_out, size_t out_cap, size_t* out_n, ndt_filter_counts* counts) {
+  if (int rc = check_filter_args(h, d_xyz, n, p, d_out, out_n)) return rc;
+  return run_filter(h, d_xyz, n, p, d_out, out_cap, out_n, counts);
+}
+
+int ndt_hip_voxel_downsample(ndt_handle* h, const float* xyz, size_t n, float resolution, float* out, size_t out_cap,
+                             size_t* out_n) {
+  // voxel_downsample_cpu alone: no distance or z test removes anything.
+  ndt_filter_params p;
+  p.min_distance = 0.0f;
+  p.max_distance = INFINITY;
+  p.min_z = -INFINITY;
+  p.max_z = INFINITY;
+  p.downsample_resolution = resolution;
+  p.downsample = 1;
+  return ndt_hip_filter_points(h, xyz, n, &p, out, out_cap, out_n, nullptr);
 }
 
 int ndt_hip_set_regularization_pose(ndt_handle* h, const double pose[6]) {

@@ -67,6 +67,10 @@ struct DeviceGrid {
 int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cell_tables, hipStream_t st,
                std::string* err);
 
+int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distance, float min_z, float max_z,
+                  int downsample, float resolution, float* d_out, size_t out_cap, size_t* out_n, uint64_t counts[4],
+                  hipStream_t st, std::string* err);
+
 // One alignment request inside a persistent launch.
 struct SlotJob {
   const float* pts;  // device AoS

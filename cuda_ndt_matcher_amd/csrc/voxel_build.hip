@@ -81,7 +81,8 @@ __device__ inline void block_minmax_store(int32_t mn[3], int32_t mx[3], uint32_t
 }
 
 __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restrict__ pts, size_t n, float res,
-                                                          int32_t* part) {
+                                                          float inv, int mul, int32_t* part) {
+  auto key = [&](float p) { return mul ? sat_i32(floorf(p * inv)) : voxel_key(p, res); };
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   const size_t S = (size_t)gridDim.x * blockDim.x;
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -97,14 +98,14 @@ __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restric
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const int32_t k = voxel_key(v[u][a], res);
+        const int32_t k = key(v[u][a]);
         mn[a] = min(mn[a], k);
         mx[a] = max(mx[a], k);
       }
   }
   for (; i < n; i += S) {
     for (int a = 0; a < 3; ++a) {
-      const int32_t k = voxel_key(pts[3 * i + a], res);
+      const int32_t k = key(pts[3 * i + a]);
       mn[a] = min(mn[a], k);
       mx[a] = max(mx[a], k);
     }
@@ -128,10 +129,19 @@ __device__ inline uint64_t compact_morton(uint32_t rx, uint32_t ry, uint32_t rz,
 
 struct KeyParams {
   float res;
+  float inv;  // 1 / res in f32 (mul keys)
+  int mul;    // 0: floor(p / res) (map voxels); 1: floor(p * inv) (downsampling)
   int32_t kmin[3];
   int bits[3];
   int passes;
 };
+
+// voxel_downsample_cpu's cell (filtering/cpu.rs:79-81): floor of the f32
+// product p * (1 / res), saturating cast.
+__device__ inline int32_t mul_key(float p, float inv) { return sat_i32(floorf(p * inv)); }
+__device__ inline int32_t key_of(float p, const KeyParams& kp) {
+  return kp.mul ? mul_key(p, kp.inv) : voxel_key(p, kp.res);
+}
 
 template <class K>
 __global__ __launch_bounds__(kBuildBlock) void keygen_hist(const float* __restrict__ pts, size_t n, KeyParams kp,
@@ -140,9 +150,9 @@ __global__ __launch_bounds__(kBuildBlock) void keygen_hist(const float* __restri
   for (int i = threadIdx.x; i < kp.passes * kRadix; i += kBuildBlock) s_hist[i] = 0;
   __syncthreads();
   auto one = [&](size_t i, float x, float y, float z) {
-    const uint32_t rx = (uint32_t)(voxel_key(x, kp.res) - kp.kmin[0]);
-    const uint32_t ry = (uint32_t)(voxel_key(y, kp.res) - kp.kmin[1]);
-    const uint32_t rz = (uint32_t)(voxel_key(z, kp.res) - kp.kmin[2]);
+    const uint32_t rx = (uint32_t)(key_of(x, kp) - kp.kmin[0]);
+    const uint32_t ry = (uint32_t)(key_of(y, kp) - kp.kmin[1]);
+    const uint32_t rz = (uint32_t)(key_of(z, kp) - kp.kmin[2]);
     const uint64_t m = compact_morton(rx, ry, rz, kp.bits[0], kp.bits[1], kp.bits[2]);
     keys[i] = (K)m;
     for (int p = 0; p < kp.passes; ++p) atomicAdd(&s_hist[p * kRadix + ((m >> (8 * p)) & 0xFF)], 1u);
@@ -636,6 +646,119 @@ hipError_t run_segments(DeviceGrid& G, size_t n, hipStream_t st, uint32_t* segme
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Sensor point filter (filtering/cpu.rs:7-58, voxel_downsample_cpu :67-100).
+
+struct FilterDev {
+  float min_dist_sq, max_dist_sq, min_z, max_z;  // squares formed in f32, as the reference does
+};
+
+// filter_points_cpu's tests in its order (cpu.rs:18-33): 0 kept, 1 removed by
+// distance, 2 removed by z.  x*x + y*y + z*z rounded after every operation
+// (no fused multiply-add), comparisons as written, so a NaN coordinate passes.
+__device__ inline int filter_class(float x, float y, float z, const FilterDev& f) {
+  float d = __fmul_rn(x, x);
+  d = __fadd_rn(d, __fmul_rn(y, y));
+  d = __fadd_rn(d, __fmul_rn(z, z));
+  if (d < f.min_dist_sq || d > f.max_dist_sq) return 1;
+  if (z < f.min_z || z > f.max_z) return 2;
+  return 0;
+}
+
+// Per tile of kSortTile points: points kept (tile_kept[tile]); the removed
+// counts are added to counts[1] (distance) and counts[2] (z).
+__global__ __launch_bounds__(kBuildBlock) void filter_count(const float* __restrict__ pts, uint32_t n, FilterDev f,
+                                                            uint32_t* __restrict__ tile_kept,
+                                                            uint32_t* __restrict__ counts) {
+  __shared__ uint32_t s[3][kBuildBlock / kWave];
+  const uint32_t base = blockIdx.x * kSortTile;
+  uint32_t k = 0, rd = 0, rz = 0;
+  for (uint32_t j = threadIdx.x; j < kSortTile; j += kBuildBlock) {
+    const uint32_t i = base + j;
+    if (i < n) {
+      const int c = filter_class(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], f);
+      k += c == 0;
+      rd += c == 1;
+      rz += c == 2;
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    k += __shfl_xor(k, off, kWave);
+    rd += __shfl_xor(rd, off, kWave);
+    rz += __shfl_xor(rz, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    s[0][threadIdx.x / kWave] = k;
+    s[1][threadIdx.x / kWave] = rd;
+    s[2][threadIdx.x / kWave] = rz;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a)
+      for (int w = 0; w < kBuildBlock / kWave; ++w) t[a] += s[a][w];
+    tile_kept[blockIdx.x] = t[0];
+    if (t[1]) atomicAdd(&counts[1], t[1]);
+    if (t[2]) atomicAdd(&counts[2], t[2]);
+  }
+}
+
+// Writes the kept points of each tile, in input order, from out + tile_off[tile].
+// Each thread owns kSortItems consecutive points; a block scan of the per-thread
+// kept counts places them.
+__global__ __launch_bounds__(kBuildBlock) void filter_compact(const float* __restrict__ pts, uint32_t n, FilterDev f,
+                                                              const uint32_t* __restrict__ tile_off,
+                                                              float* __restrict__ out) {
+  __shared__ uint32_t s[kBuildBlock];
+  const uint32_t base = blockIdx.x * kSortTile + threadIdx.x * kSortItems;
+  uint32_t keep = 0;  // bit k: point base + k is kept
+#pragma unroll
+  for (int k = 0; k < kSortItems; ++k) {
+    const uint32_t i = base + k;
+    if (i < n && filter_class(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], f) == 0) keep |= 1u << k;
+  }
+  const uint32_t c = (uint32_t)__popc(keep);
+  s[threadIdx.x] = c;
+  __syncthreads();
+  for (int off = 1; off < kBuildBlock; off <<= 1) {
+    const uint32_t t = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  uint32_t o = tile_off[blockIdx.x] + s[threadIdx.x] - c;
+  for (int k = 0; k < kSortItems; ++k)
+    if (keep & (1u << k)) {
+      const uint32_t i = base + k;
+      out[3 * o] = pts[3 * i];
+      out[3 * o + 1] = pts[3 * i + 1];
+      out[3 * o + 2] = pts[3 * i + 2];
+      ++o;
+    }
+}
+
+// One voxel per thread: f64 sums of its points in input order (the stable
+// sort keeps it), centroid (sum / n) rounded to f32 (cpu.rs:82-98).
+__global__ __launch_bounds__(kBuildBlock) void voxel_centroids(const float* __restrict__ pts,
+                                                               const uint32_t* __restrict__ order,
+                                                               const uint32_t* __restrict__ seg_start, uint32_t segs,
+                                                               uint32_t n, float* __restrict__ out) {
+  const uint32_t s = blockIdx.x * kBuildBlock + threadIdx.x;
+  if (s >= segs) return;
+  const uint32_t b = seg_start[s], e = s + 1 < segs ? seg_start[s + 1] : n;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (uint32_t i = b; i < e; ++i) {
+    const uint32_t j = order[i];
+    sx += (double)pts[3 * j];
+    sy += (double)pts[3 * j + 1];
+    sz += (double)pts[3 * j + 2];
+  }
+  const double cnt = (double)(e - b);
+  out[3 * s] = (float)(sx / cnt);
+  out[3 * s + 1] = (float)(sy / cnt);
+  out[3 * s + 2] = (float)(sz / cnt);
+}
+
 uint32_t next_pow2(uint64_t v) {
   uint64_t p = 1;
   while (p < v) p <<= 1;
@@ -691,7 +814,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
     if (count) *count = (uint32_t)c;
   };
   const int kb_blocks = grid_for(n, kBuildBlock, kMaxBoundBlocks);
-  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, part);
+  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, 1.0f / res, 0, part);
   if ((e = hipMemcpyAsync(G.host_part, part, (size_t)kb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
       hipSuccess)
     return fail(e, "bounds copy");
@@ -700,6 +823,8 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   reduce_parts(kb_blocks, kb, nullptr);
   KeyParams kp;
   kp.res = res;
+  kp.inv = 1.0f / res;
+  kp.mul = 0;
   int total_bits = 0;
   for (int a = 0; a < 3; ++a) {
     const int64_t span = (int64_t)kb[3 + a] - (int64_t)kb[a];
@@ -830,6 +955,145 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
     *err = "No voxels created from target points (too sparse?)";
     return -4;
   }
+  return 0;
+}
+
+// Sensor point filter on the device: G.points (n points) -> d_out.  The kept
+// points (input order) or, with downsampling, one centroid per occupied cell
+// (in the cells' key order).  Counts: [0] kept before downsampling, [1]
+// removed by distance, [2] removed by z, [3] removed by downsampling.
+// G is a grid used only for its scratch buffers (never a map).
+int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distance, float min_z, float max_z,
+                  int downsample, float resolution, float* d_out, size_t out_cap, size_t* out_n, uint64_t counts[4],
+                  hipStream_t st, std::string* err) {
+  auto fail = [&](hipError_t e, const char* what) {
+    *err = std::string(what) + ": " + hipGetErrorString(e);
+    return -5;
+  };
+  for (int i = 0; i < 4; ++i) counts[i] = 0;
+  *out_n = 0;
+  if (n == 0) return 0;
+  if (n >= (1ull << 30)) {
+    *err = "point cloud exceeds 2^30 points";
+    return -7;
+  }
+  hipError_t e;
+  if (!G.host_scalars &&
+      (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_scalars), 64 * sizeof(uint32_t), hipHostMallocDefault)) !=
+          hipSuccess)
+    return fail(e, "pinned alloc");
+  if (!G.host_part &&
+      (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_part), (size_t)kMaxBoundBlocks * 8 * sizeof(int32_t),
+                         hipHostMallocDefault)) != hipSuccess)
+    return fail(e, "pinned alloc");
+  const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  if ((e = G.scalars.reserve(64 * sizeof(uint32_t))) != hipSuccess) return fail(e, "alloc");
+  if ((e = G.tile_heads.reserve((size_t)tiles * 4)) != hipSuccess) return fail(e, "alloc");
+  uint32_t* scalars = reinterpret_cast<uint32_t*>(G.scalars.ptr);
+  uint32_t* tile_kept = reinterpret_cast<uint32_t*>(G.tile_heads.ptr);
+  FilterDev f;
+  f.min_dist_sq = min_distance * min_distance;
+  f.max_dist_sq = max_distance * max_distance;
+  f.min_z = min_z;
+  f.max_z = max_z;
+  const bool ds = downsample && !(resolution <= 0.0f);  // cpu.rs:68: resolution <= 0 keeps the points
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, st, scalars, (size_t)64, 0u);
+  hipLaunchKernelGGL(filter_count, dim3(tiles), dim3(kBuildBlock), 0, st, G.points, (uint32_t)n, f, tile_kept,
+                     scalars);
+  hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_kept, tiles, scalars);
+  if ((e = hipMemcpyAsync(G.host_scalars, scalars, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return fail(e, "count copy");
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "filter sync");
+  const uint32_t kept = G.host_scalars[0];
+  counts[0] = kept;
+  counts[1] = G.host_scalars[1];
+  counts[2] = G.host_scalars[2];
+  if (!ds) {
+    if (out_cap < kept) {
+      *out_n = kept;
+      *err = "output capacity below the kept point count";
+      return -1;
+    }
+    if (kept) hipLaunchKernelGGL(filter_compact, dim3(tiles), dim3(kBuildBlock), 0, st, G.points, (uint32_t)n, f,
+                                 tile_kept, d_out);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "filter sync");
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "filter kernels");
+    *out_n = kept;
+    return 0;
+  }
+  if (kept == 0) return 0;
+  if ((e = G.owned_points.reserve((size_t)kept * 12)) != hipSuccess) return fail(e, "alloc");
+  float* compact = reinterpret_cast<float*>(G.owned_points.ptr);
+  hipLaunchKernelGGL(filter_compact, dim3(tiles), dim3(kBuildBlock), 0, st, G.points, (uint32_t)n, f, tile_kept,
+                     compact);
+  // cells of the kept points, as voxel_downsample_cpu forms them
+  if ((e = G.block_part.reserve((size_t)kMaxBoundBlocks * 8 * sizeof(int32_t))) != hipSuccess)
+    return fail(e, "alloc");
+  int32_t* part = reinterpret_cast<int32_t*>(G.block_part.ptr);
+  const float inv = 1.0f / resolution;
+  const int kb_blocks = grid_for(kept, kBuildBlock, kMaxBoundBlocks);
+  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, compact, (size_t)kept, resolution, inv,
+                     1, part);
+  if ((e = hipMemcpyAsync(G.host_part, part, (size_t)kb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
+      hipSuccess)
+    return fail(e, "bounds copy");
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "bounds sync");
+  int32_t kb[6];
+  for (int a = 0; a < 3; ++a) {
+    kb[a] = INT32_MAX;
+    kb[3 + a] = INT32_MIN;
+  }
+  for (int b = 0; b < kb_blocks; ++b)
+    for (int a = 0; a < 3; ++a) {
+      kb[a] = std::min(kb[a], G.host_part[8 * b + a]);
+      kb[3 + a] = std::max(kb[3 + a], G.host_part[8 * b + 3 + a]);
+    }
+  KeyParams kp;
+  kp.res = resolution;
+  kp.inv = inv;
+  kp.mul = 1;
+  int total_bits = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t span = (int64_t)kb[3 + a] - (int64_t)kb[a];
+    if (kb[a] < -kCoordBias || kb[3 + a] >= kCoordBias || span >= (1ll << 21)) {
+      *err = "point extent exceeds 2^20 cells from the origin on some axis at this resolution";
+      return -7;
+    }
+    int bits = 0;
+    while (((int64_t)1 << bits) <= span) ++bits;
+    kp.kmin[a] = kb[a];
+    kp.bits[a] = bits;
+    total_bits += bits;
+  }
+  kp.passes = std::max(1, (total_bits + 7) / 8);
+  const bool key64 = total_bits > 32;
+  const size_t ksz = key64 ? 8 : 4;
+  if ((e = G.sort_keys_a.reserve((size_t)kept * ksz)) != hipSuccess) return fail(e, "alloc keys");
+  if ((e = G.sort_keys_b.reserve((size_t)kept * ksz)) != hipSuccess) return fail(e, "alloc keys");
+  if ((e = G.sort_vals_a.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc vals");
+  if ((e = G.sort_vals_b.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc vals");
+  if ((e = G.hist.reserve((8 * kRadix + 8) * 4)) != hipSuccess) return fail(e, "alloc hist");
+  const uint32_t ktiles = (uint32_t)((kept + kSortTile - 1) / kSortTile);
+  if ((e = G.status.reserve((size_t)kp.passes * ktiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc status");
+  if ((e = G.tile_heads.reserve((size_t)std::max(tiles, ktiles) * 4)) != hipSuccess) return fail(e, "alloc tiles");
+  if ((e = G.seg_start.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc segs");
+  G.points = compact;
+  e = key64 ? run_sort<uint64_t>(G, kp, kept, st) : run_sort<uint32_t>(G, kp, kept, st);
+  if (e != hipSuccess) return fail(e, "radix sort");
+  uint32_t segs = 0;
+  e = key64 ? run_segments<uint64_t>(G, kept, st, &segs) : run_segments<uint32_t>(G, kept, st, &segs);
+  if (e != hipSuccess) return fail(e, "segments");
+  if (out_cap < segs) {
+    *out_n = segs;
+    *err = "output capacity below the downsampled point count";
+    return -1;
+  }
+  hipLaunchKernelGGL(voxel_centroids, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st, compact,
+                     G.sorted_vals, reinterpret_cast<const uint32_t*>(G.seg_start.ptr), segs, kept, d_out);
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "filter sync");
+  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "filter kernels");
+  counts[3] = kept - segs;
+  *out_n = segs;
   return 0;
 }
 

@@ -140,6 +140,14 @@ def dense_scan(name: str, seed: int, leaf: float) -> np.ndarray:
     return voxel_downsample(raw, leaf)
 
 
+def raw_scan(name: str, seed: int) -> np.ndarray:
+    """The workload's lidar scan before downsampling (sensor frame)."""
+    extent, nb, npl, _, beams, _, _, _, _ = WORKLOADS[name]
+    wl = make_workload(name, seed)
+    scene = Scene(seed, extent, nb, npl)
+    return scene.lidar(wl.true_pose[0], wl.true_pose[1], wl.true_pose[2], wl.true_pose[5], beams, seed=seed + 1)
+
+
 def sweep_scans(seed: int, sizes, leaf: float = 0.1):
     """Scans of the given sizes over the small workload's map (the precision
     sweep): one 128-beam scan from the small workload's sensor pose,

@@ -229,6 +229,53 @@ int ndt_hip_clear_regularization_pose(ndt_handle* h);
 int ndt_hip_laplace_covariance_xy(const double hess[36], double cov[4]);
 int ndt_hip_laplace_covariance_6x6(const double hess[36], double cov[36]);
 
+/* Sensor point filter: FilterParams (src/ndt_cuda/src/filtering/mod.rs:13-31,
+ * defaults :22-31).  min/max distance from the sensor origin, min/max z, and
+ * an optional voxel-grid downsampling (downsample = 0 is the reference's
+ * None).  Replaces GpuPointFilter::filter / filter_points_cpu
+ * (filtering/mod.rs:61-230, cpu.rs:7-58) as called by
+ * cuda_ndt_matcher/src/pointcloud.rs:59-95 (filter_sensor_points). */
+typedef struct ndt_filter_params {
+  float min_distance;
+  float max_distance;
+  float min_z;
+  float max_z;
+  float downsample_resolution;
+  int32_t downsample;
+} ndt_filter_params;
+
+/* FilterResult's counts (filtering/mod.rs:34-41) plus the points kept by the
+ * distance and z tests (before downsampling). */
+typedef struct ndt_filter_counts {
+  uint64_t kept;
+  uint64_t removed_by_distance;
+  uint64_t removed_by_z;
+  uint64_t removed_by_downsampling;
+} ndt_filter_counts;
+
+void ndt_hip_filter_params_default(ndt_filter_params* p);
+
+/* filter_points_cpu (filtering/cpu.rs:7-58) on the GPU.  Each removed point is
+ * counted under the first test it fails (distance, then z; a NaN coordinate
+ * passes both, as in the reference's CPU path).  Without downsampling the kept
+ * points come out in input order; with it, one centroid per occupied cell of
+ * floor(p * (1/res)) (f64 sums in input order, rounded to f32), in the cells'
+ * key order (the reference returns HashMap order).  out (3*out_cap floats)
+ * must hold the result: a smaller out_cap returns NDT_ERR_INVALID_ARGUMENT with
+ * the needed count in *out_n; n >= 2^30, or cells beyond 2^20 from the origin
+ * on some axis, return NDT_ERR_CAPACITY.  Needs no target; the handle's map is
+ * untouched.
+ * The _device form reads d_xyz and writes d_out in device memory (a filtered
+ * scan can go straight to ndt_hip_align_device). */
+int ndt_hip_filter_points(ndt_handle* h, const float* xyz, size_t n, const ndt_filter_params* p, float* out,
+                          size_t out_cap, size_t* out_n, ndt_filter_counts* counts);
+int ndt_hip_filter_points_device(ndt_handle* h, const float* d_xyz, size_t n, const ndt_filter_params* p,
+                                 float* d_out, size_t out_cap, size_t* out_n, ndt_filter_counts* counts);
+/* voxel_downsample_cpu (filtering/cpu.rs:67-100): resolution <= 0 returns the
+ * points unchanged. */
+int ndt_hip_voxel_downsample(ndt_handle* h, const float* xyz, size_t n, float resolution, float* out, size_t out_cap,
+                             size_t* out_n);
+
 /* Continuous-batching server.  No counterpart in the reference: it serves
  * the multi-scan path (align_parallel_scans, ndt.rs:1224-1320) as a stream of
  * requests.  One persistent launch holds `slots` slots of `wg_per_slot`

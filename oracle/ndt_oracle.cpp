@@ -26,6 +26,8 @@
 #include <unordered_map>
 #include <utility>
 #include <vector>
+#include <tuple>
+#include <map>
 
 namespace {
 
@@ -1203,6 +1205,71 @@ void oracle_matrix_to_pose(const double m12[12], double pose[6]) {
   pose[3] = roll;
   pose[4] = pitch;
   pose[5] = yaw;
+}
+
+// filter_points_cpu (src/ndt_cuda/src/filtering/cpu.rs:7-58): distance test
+// on x*x + y*y + z*z in f32 (every operation rounded), then the z test; the
+// kept points in input order.  With downsample != 0 and resolution > 0,
+// voxel_downsample_cpu (:67-100): cell floor(p * (1 / res)) in f32 with a
+// saturating cast, f64 sums in input order, centroid (sum / n) as f32.  The
+// reference emits the cells in HashMap order; here they come out in
+// (x, y, z) lexicographic order of the cell.  counts: kept (before
+// downsampling), removed by distance, by z, by downsampling.
+size_t oracle_filter_points(const float* xyz, size_t n, float min_distance, float max_distance, float min_z,
+                            float max_z, int32_t downsample, float resolution, float* out, uint64_t counts[4]) {
+  const float min_sq = min_distance * min_distance, max_sq = max_distance * max_distance;
+  std::vector<float> kept;
+  kept.reserve(3 * n);
+  uint64_t rd = 0, rz = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    const float d = x * x + y * y + z * z;  // built with -ffp-contract=off
+    if (d < min_sq || d > max_sq) {
+      ++rd;
+      continue;
+    }
+    if (z < min_z || z > max_z) {
+      ++rz;
+      continue;
+    }
+    kept.push_back(x);
+    kept.push_back(y);
+    kept.push_back(z);
+  }
+  const size_t k = kept.size() / 3;
+  counts[0] = k;
+  counts[1] = rd;
+  counts[2] = rz;
+  counts[3] = 0;
+  if (!downsample || resolution <= 0.0f || k == 0) {
+    std::copy(kept.begin(), kept.end(), out);
+    return k;
+  }
+  const float inv = 1.0f / resolution;
+  struct Acc {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    size_t c = 0;
+  };
+  std::map<std::tuple<int32_t, int32_t, int32_t>, Acc> cells;
+  for (size_t i = 0; i < k; ++i) {
+    const float* q = &kept[3 * i];
+    Acc& a = cells[std::make_tuple(sat_i32(std::floor(q[0] * inv)), sat_i32(std::floor(q[1] * inv)),
+                                   sat_i32(std::floor(q[2] * inv)))];
+    a.sx += (double)q[0];
+    a.sy += (double)q[1];
+    a.sz += (double)q[2];
+    ++a.c;
+  }
+  size_t o = 0;
+  for (const auto& kv : cells) {
+    const double c = (double)kv.second.c;
+    out[3 * o] = (float)(kv.second.sx / c);
+    out[3 * o + 1] = (float)(kv.second.sy / c);
+    out[3 * o + 2] = (float)(kv.second.sz / c);
+    ++o;
+  }
+  counts[3] = k - o;
+  return o;
 }
 
 }  // extern "C"

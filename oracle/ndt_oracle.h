@@ -29,6 +29,9 @@
  *   oracle_oscillation        the reference's cases
  *   oracle_laplace_xy         closed-form 2x2 inverse
  *   oracle_pose_to_matrix/matrix_to_pose   round trip and rotation-order cases
+ *   oracle_filter_points      the reference's filter and downsampling cases
+ *                             (filtering/cpu.rs tests), brute-force numpy
+ *                             grouping with f64 sums in input order
  * Parity unpinned (the reference is Rust and cannot be built here, so these
  * third-party results are restated, not compared bit for bit): the rounding
  * of nalgebra 0.33's symmetric_eigen, SVD solve and Euler conversions, and
@@ -159,6 +162,12 @@ uint32_t oracle_oscillation(const double* translations, size_t k);
 
 /* estimate_xy_covariance_by_laplace (cuda_ndt_matcher/src/covariance.rs:82-98). */
 void oracle_laplace_xy(const double hess36[36], double cov4[4]);
+
+/* filter_points_cpu + voxel_downsample_cpu (filtering/cpu.rs:7-100); out holds
+ * 3n floats; returns the number of points written (cells in lexicographic
+ * order when downsampling). */
+size_t oracle_filter_points(const float* xyz, size_t n, float min_distance, float max_distance, float min_z,
+                            float max_z, int32_t downsample, float resolution, float* out, uint64_t counts[4]);
 
 /* Isometry helpers restated from nalgebra 0.33 (used by optimization/types.rs:156-174). */
 void oracle_pose_to_matrix(const double pose[6], double m12[12]);

@@ -99,6 +99,10 @@ def _load():
     lib.oracle_laplace_xy.argtypes = [f64p, f64p]
     lib.oracle_pose_to_matrix.argtypes = [f64p, f64p]
     lib.oracle_matrix_to_pose.argtypes = [f64p, f64p]
+    u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+    lib.oracle_filter_points.restype = C.c_size_t
+    lib.oracle_filter_points.argtypes = [f32p, C.c_size_t, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
+                                         C.c_float, f32p, u64p]
     return lib
 
 
@@ -313,3 +317,22 @@ def matrix_to_pose(m) -> np.ndarray:
     out = np.zeros(6)
     lib().oracle_matrix_to_pose(np.ascontiguousarray(m, np.float64).reshape(12), out)
     return out
+
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+def filter_points(points, min_distance=0.0, max_distance=FLT_MAX, min_z=-FLT_MAX, max_z=FLT_MAX,
+                  downsample_resolution=None):
+    """filter_points_cpu (filtering/cpu.rs:7-58) with voxel_downsample_cpu
+    (:67-100) when downsample_resolution is not None.  Returns the points
+    ((k, 3) float32; downsampled cells in lexicographic cell order) and the
+    counts (kept before downsampling, removed by distance, by z, by
+    downsampling)."""
+    p = _pts(points)
+    out = np.zeros((max(1, p.shape[0]), 3), np.float32)
+    counts = np.zeros(4, np.uint64)
+    ds = downsample_resolution is not None
+    k = lib().oracle_filter_points(p.reshape(-1), p.shape[0], min_distance, max_distance, min_z, max_z, int(ds),
+                                   float(downsample_resolution) if ds else 0.0, out.reshape(-1), counts)
+    return out[:k].copy(), [int(c) for c in counts]

@@ -41,7 +41,8 @@ def test_struct_layouts(tmp_path):
 
     assert C.sizeof(_lib.NdtResult) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8
     mirrors = {"ndt_config": _lib.NdtConfig, "ndt_result": _lib.NdtResult, "ndt_timing": _lib.NdtTiming,
-               "ndt_phase_profile": _lib.NdtPhaseProfile, "ndt_tuning_entry": _lib.NdtTuningEntry}
+               "ndt_phase_profile": _lib.NdtPhaseProfile, "ndt_tuning_entry": _lib.NdtTuningEntry,
+               "ndt_filter_params": _lib.NdtFilterParams, "ndt_filter_counts": _lib.NdtFilterCounts}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ndt_hip.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
