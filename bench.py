@@ -349,6 +349,93 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     return out
 
 
+def ingest_bench(seed, with_cpu=True, reps=20):
+    """Either side of the alignment.  (1) The sensor point filter on the
+    128-beam scan of the 5M-point workload (about 230k points, device
+    resident): distance 1-100 m, z -3..40 m, 0.3 m voxel downsampling, through
+    filter_points_device; wall time per call (it includes the filter's host
+    synchronisations) and the oracle's single-core time on the same cloud.
+    (2) Background map rebuild: alignments of the 200k-point workload's scan on
+    the active matcher while the standby one builds the 5M-point map."""
+    import torch
+
+    from cuda_ndt_matcher_amd import NdtScanMatcherConfig, synth
+    from cuda_ndt_matcher_amd.dual import DualMatcher
+    from cuda_ndt_matcher_amd.filtering import FilterParams, GpuPointFilter
+
+    raw = synth.raw_scan("full", seed)
+    prm = FilterParams(min_distance=1.0, max_distance=100.0, min_z=-3.0, max_z=40.0, downsample_resolution=0.3)
+    f = GpuPointFilter()
+    d_in = torch.from_numpy(raw).to("cuda:0")
+    d_out = torch.empty_like(d_in)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        k, res = f.filter_device(d_in.data_ptr(), raw.shape[0], prm, d_out.data_ptr(), raw.shape[0])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        k, res = f.filter_device(d_in.data_ptr(), raw.shape[0], prm, d_out.data_ptr(), raw.shape[0])
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    kept = raw.shape[0] - res.removed_by_distance - res.removed_by_z
+    out = {"filter": {"input_points": int(raw.shape[0]), "kept": int(kept), "output_points": int(k),
+                      "ms_per_call": ms, "points_per_s": raw.shape[0] / (ms * 1e-3)}}
+    if with_cpu:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        n_cpu = 0
+        while True:
+            O.filter_points(raw, 1.0, 100.0, -3.0, 40.0, 0.3)
+            n_cpu += 1
+            if time.perf_counter() - t0 > 3.0:
+                break
+        cms = (time.perf_counter() - t0) * 1e3 / n_cpu
+        out["filter"]["cpu_baseline"] = {"value": cms, "unit": "ms per call", "cores": 1, "kind": "port",
+                                         "sample": f"{n_cpu} oracle calls (C++ restatement of filter_points_cpu + "
+                                                   f"voxel_downsample_cpu) on the same {raw.shape[0]}-pt cloud"}
+    f.close()
+    print(f"[bench] filter: {raw.shape[0]} -> {kept} -> {k} pts in {ms:.3f} ms per call"
+          + (f" (oracle {out['filter']['cpu_baseline']['value']:.1f} ms)" if with_cpu else ""), file=sys.stderr,
+          flush=True)
+
+    small = synth.make_workload("small", seed)
+    big = synth.make_workload("full", seed)
+    d = DualMatcher(NdtScanMatcherConfig(resolution=2.0, max_iterations=30, trans_epsilon=0.01))
+    d.set_target(small.map_points)
+    for _ in range(3):
+        d.align_batch(small.scan, [small.guess])
+    t0 = time.perf_counter()
+    idle = 0
+    while time.perf_counter() - t0 < 0.5:
+        d.align_batch(small.scan, [small.guess])
+        idle += 1
+    idle_rate = idle / (time.perf_counter() - t0)
+    rebuild = {}
+    # 5M-point map onto the standby handle (its first build allocates), the
+    # 200k map back onto the other, then the 5M map again (buffers reused);
+    # the small scan is aligned only while the active handle holds its map.
+    for label, pts in (("first", big.map_points), (None, small.map_points), ("warm", big.map_points)):
+        d.start_background_update(pts)
+        during = 0
+        t0 = time.perf_counter()
+        while d.is_update_in_progress() and not d.swap_if_ready():
+            if label is None:
+                time.sleep(0.001)
+                continue
+            d.align_batch(small.scan, [small.guess])
+            during += 1
+        el = time.perf_counter() - t0
+        if label is not None:
+            rebuild[label] = {"update_ms": d.get_status().last_update_ms, "aligns_during": during,
+                              "aligns_per_s_during": during / el if el > 0 else 0.0}
+    d.close()
+    out["dual_rebuild"] = {"map_points": int(big.map_points.shape[0]), "aligns_per_s_idle": idle_rate, **rebuild}
+    for label in ("first", "warm"):
+        print(f"[bench] dual matcher: {idle_rate:.0f} aligns/s idle; {label} 5M-point rebuild "
+              f"{rebuild[label]['update_ms']:.1f} ms with {rebuild[label]['aligns_during']} alignments meanwhile "
+              f"({rebuild[label]['aligns_per_s_during']:.0f}/s)", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(wl, budget_s=12.0):
     from oracle import oracle as O
 
@@ -418,6 +505,8 @@ def main():
     ap.add_argument("--learned-epochs", type=int, default=60, help="training epochs for --learned")
     ap.add_argument("--serve", action="store_true",
                     help="also serve a Poisson stream of 1k-40k-point requests (continuous batching)")
+    ap.add_argument("--ingest", action="store_true",
+                    help="also time the sensor point filter and alignments during a background map rebuild")
     ap.add_argument("--batching", action="store_true",
                     help="also compare direct launches, graph replay and the autotuner on three launch shapes")
     args = ap.parse_args()
@@ -549,6 +638,8 @@ def main():
                                                    with_cpu=not args.no_cpu)
     if args.batching:
         extra["batching"] = batching_bench(wl, args.seed)
+    if args.ingest:
+        extra["ingest"] = ingest_bench(args.seed, with_cpu=not args.no_cpu)
     if args.precision_sweep:
         extra["precision_sweep"] = precision_sweep(args.seed, max(3, args.steps // 5))
     line.update(extra)

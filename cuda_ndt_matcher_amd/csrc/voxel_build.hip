@@ -747,7 +747,30 @@ __global__ __launch_bounds__(kBuildBlock) void voxel_centroids(const float* __re
   if (s >= segs) return;
   const uint32_t b = seg_start[s], e = s + 1 < segs ? seg_start[s + 1] : n;
   double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (uint32_t i = b; i < e; ++i) {
+  uint32_t i = b;
+  // Sixteen points per step, all loads before the (in-order) additions: a
+  // dense cell near the sensor holds hundreds of points, and one dependent
+  // gather pair per point would serialise their latencies.
+  constexpr int U = 16;
+  for (; i + U <= e; i += U) {
+    uint32_t j[U];
+    float x[U], y[U], z[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) j[u] = order[i + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      x[u] = pts[3 * j[u]];
+      y[u] = pts[3 * j[u] + 1];
+      z[u] = pts[3 * j[u] + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      sx += (double)x[u];
+      sy += (double)y[u];
+      sz += (double)z[u];
+    }
+  }
+  for (; i < e; ++i) {
     const uint32_t j = order[i];
     sx += (double)pts[3 * j];
     sy += (double)pts[3 * j + 1];
