@@ -296,19 +296,28 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     # of U(+-2 m)), z at its 0.1 m floor, roll/pitch 1 deg; yaw is uniform.
     tpe_cov = np.diag([4.0 / 3.0, 4.0 / 3.0, 0.0, math.radians(1) ** 2, math.radians(1) ** 2, 0.0]).ravel()
     tpe = {1: [0, 0.0], 10: [0, 0.0]}
+    # the plain search with the learned search's second launch, at 200 and 1000 particles
+    plain_pol = {200: [0, 0.0], 1000: [0, 0.0]}
+    stage = {}
     for k, smp in enumerate(test):
         t = time.perf_counter()
         rp = proposal.plain_search(m, smp, 200, seed + k)
         plain_t += time.perf_counter() - t
         plain_ok += close(rp.pose_vector, ref[k])
+        for npart, acc in plain_pol.items():
+            t = time.perf_counter()
+            rq = proposal.plain_search(m, smp, npart, seed + k, polish=4)
+            acc[1] += time.perf_counter() - t
+            acc[0] += close(rq.pose_vector, ref[k])
         for gb, acc in tpe.items():
             t = time.perf_counter()
             rt = estimate_initial_pose(m, smp.scan, smp.true_pose, tpe_cov, guided_batch=gb, seed=seed + k)
             acc[1] += time.perf_counter() - t
             acc[0] += close(rt.pose, ref[k])
         t = time.perf_counter()
-        r, fb = proposal.learned_search(m, net, frame, smp, tb[k:k + 1], fallback=lambda s_, k=k: proposal.plain_search(
-            m, s_, 200, seed + k))
+        # the descriptor is formed inside the timed region, from the scan alone
+        r, fb = proposal.learned_search(m, net, frame, smp, None, fallback=lambda s_, k=k: proposal.plain_search(
+            m, s_, 200, seed + k), times=stage, form_descriptor=True)
         direct += (not fb) and close(r.pose_vector, ref[k])
         learn_t += time.perf_counter() - t
         learn_ok += close(r.pose_vector, ref[k])
@@ -335,6 +344,9 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
                        "success_without_fallback": direct / n_test},
            "tpe": {f"guided_batch_{gb}": {"success_rate": acc[0] / n_test, "solves_per_s": n_test / acc[1]}
                    for gb, acc in tpe.items()},
+           "plain_with_polish": {f"particles_{npart}": {"success_rate": acc[0] / n_test, "solves_per_s": n_test / acc[1]}
+                                 for npart, acc in plain_pol.items()},
+           "learned_stage_ms_per_solve": {key: 1e3 * v / n_test for key, v in stage.items()},
            "speedup": plain_t / learn_t,
            "best_hypothesis_err_m_p50_p90": [float(np.percentile(best_xy, 50)), float(np.percentile(best_xy, 90))],
            "best_hypothesis_err_deg_p50_p90": [float(np.percentile(best_yaw, 50)), float(np.percentile(best_yaw, 90))],
@@ -343,7 +355,10 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
           f"training); plain {plain_ok}/{n_test} at {n_test / plain_t:.1f} solves/s, learned {learn_ok}/{n_test} at "
           f"{n_test / learn_t:.1f} solves/s ({fallbacks} fallbacks, {direct} right without one, {agree} agree with plain); "
           f"TPE {tpe[1][0]}/{n_test} at {n_test / tpe[1][1]:.1f} solves/s (one guided particle per launch), "
-          f"{tpe[10][0]}/{n_test} at {n_test / tpe[10][1]:.1f} (ten per launch); best hypothesis p50 "
+          f"{tpe[10][0]}/{n_test} at {n_test / tpe[10][1]:.1f} (ten per launch); plain+polish "
+          f"{plain_pol[200][0]}/{n_test} at {n_test / plain_pol[200][1]:.1f} (200), {plain_pol[1000][0]}/{n_test} at "
+          f"{n_test / plain_pol[1000][1]:.1f} (1000); learned stages ms/solve "
+          f"{ {key: round(1e3 * v / n_test, 3) for key, v in stage.items()} }; best hypothesis p50 "
           f"{out['best_hypothesis_err_m_p50_p90'][0]:.2f} m {out['best_hypothesis_err_deg_p50_p90'][0]:.1f} deg, "
           f"{near}/{n_test} within 1 m / 15 deg", file=sys.stderr, flush=True)
     return out

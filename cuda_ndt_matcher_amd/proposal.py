@@ -295,7 +295,8 @@ def propose(net: ProposalNet, frame: Frame, bevs, k: int) -> np.ndarray:
 
 
 def learned_search(matcher, net: ProposalNet, frame: Frame, sample: Sample, bev_row, k: int = HYPOTHESES,
-                   nvtl_threshold: float = 2.3, polish: int = 4, fallback=None):
+                   nvtl_threshold: float = 2.3, polish: int = 4, fallback=None, times: Optional[dict] = None,
+                   form_descriptor: bool = False):
     """Proposals -> one batched refinement of the top k.  When the best
     refined pose (by NVTL) has not converged, the `polish` best refined poses
     run once more from where they stopped (one more batched launch: a
@@ -303,26 +304,50 @@ def learned_search(matcher, net: ProposalNet, frame: Frame, sample: Sample, bev_
     before it converges).  The plain search (fallback(sample)) runs when the
     best NVTL is still below the threshold.  Height comes from the prior
     (the sample's pose), roll and pitch start at zero.
+    With form_descriptor the descriptor is formed here from the sample's scan
+    (bev_row is ignored).  `times`, if given, accumulates seconds per stage
+    (descriptor, proposal, refine, polish, fallback).
     Returns (best AlignResult, used_fallback)."""
+    import time
+
+    def lap(key, t0):
+        if times is not None:
+            times[key] = times.get(key, 0.0) + time.perf_counter() - t0
+        return time.perf_counter()
+
+    t = time.perf_counter()
+    if form_descriptor:
+        bev_row = bev_batch([sample.scan])
+        t = lap("descriptor", t)
     hyp = propose(net, frame, bev_row, k)[0]
+    t = lap("proposal", t)
     z = sample.true_pose[2]
     poses = [np.array([h[0], h[1], z, 0.0, 0.0, h[2]]) for h in hyp]
     res = sorted(matcher.align_batch(sample.scan, poses), key=lambda r: -r.nvtl)
+    t = lap("refine", t)
     if polish > 0 and not res[0].converged:
         again = matcher.align_batch(sample.scan, [r.pose_vector for r in res[:polish]])
         res = sorted(again + res, key=lambda r: -r.nvtl)
+        t = lap("polish", t)
     best = res[0]
     if best.nvtl < nvtl_threshold and fallback is not None:
-        return fallback(sample), True
+        out = fallback(sample), True
+        lap("fallback", t)
+        return out
     return best, False
 
 
-def plain_search(matcher, sample: Sample, particles: int, seed: int):
+def plain_search(matcher, sample: Sample, particles: int, seed: int, polish: int = 0):
     """Particles around the true pose: translation U(+-2 m), yaw U(+-pi),
-    roll/pitch N(0, 1 deg); all refined in one launch; best NVTL wins."""
+    roll/pitch N(0, 1 deg); all refined in one launch; best NVTL wins.  With
+    polish > 0 the polish best run once more when the best has not converged
+    (the learned search's second launch)."""
     r = np.random.default_rng(seed)
     p0 = sample.true_pose
     poses = [p0 + np.array([r.uniform(-2, 2), r.uniform(-2, 2), 0.0, r.normal(0, math.radians(1)),
                             r.normal(0, math.radians(1)), r.uniform(-math.pi, math.pi)]) for _ in range(particles)]
-    res = matcher.align_batch(sample.scan, poses)
-    return max(res, key=lambda x: x.nvtl)
+    res = sorted(matcher.align_batch(sample.scan, poses), key=lambda x: -x.nvtl)
+    if polish > 0 and not res[0].converged:
+        again = matcher.align_batch(sample.scan, [x.pose_vector for x in res[:polish]])
+        res = sorted(again + res, key=lambda x: -x.nvtl)
+    return res[0]

@@ -157,3 +157,22 @@ def test_plain_search_particles():
     m2 = _Matcher([lambda k, p: _Res(nvtl=0.0, converged=True, pose_vector=p)])
     proposal.plain_search(m2, s, 200, seed=9)
     np.testing.assert_array_equal(np.array(m2.calls[0]), sent)
+
+
+def test_plain_search_polish_pass():
+    """With polish, the four best particles (by NVTL) restart from where they
+    stopped when the best has not converged; the best of both launches wins.
+    A converged best needs no second launch."""
+    from cuda_ndt_matcher_amd import proposal
+
+    first = lambda k, p: _Res(nvtl=0.01 * k, converged=False, pose_vector=p + 1.0)  # noqa: E731
+    again = lambda k, p: _Res(nvtl=5.0 - k, converged=True, pose_vector=p)  # noqa: E731
+    m = _Matcher([first, again])
+    r = proposal.plain_search(m, _sample(), 20, seed=3, polish=4)
+    sent = np.array(m.calls[0])
+    assert len(m.calls) == 2
+    np.testing.assert_allclose(np.array(m.calls[1]), sent[[19, 18, 17, 16]] + 1.0)
+    assert r.nvtl == 5.0 and np.array_equal(r.pose_vector, sent[19] + 1.0)
+    m2 = _Matcher([lambda k, p: _Res(nvtl=0.01 * k, converged=True, pose_vector=p)])
+    proposal.plain_search(m2, _sample(), 20, seed=3, polish=4)
+    assert len(m2.calls) == 1
