@@ -299,6 +299,7 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     # the plain search with the learned search's second launch, at 200 and 1000 particles
     plain_pol = {200: [0, 0.0], 1000: [0, 0.0]}
     stage = {}
+    query = proposal.QueryScan()
     for k, smp in enumerate(test):
         t = time.perf_counter()
         rp = proposal.plain_search(m, smp, 200, seed + k)
@@ -317,7 +318,7 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
         t = time.perf_counter()
         # the descriptor is formed inside the timed region, from the scan alone
         r, fb = proposal.learned_search(m, net, frame, smp, None, fallback=lambda s_, k=k: proposal.plain_search(
-            m, s_, 200, seed + k), times=stage, form_descriptor=True)
+            m, s_, 200, seed + k), times=stage, query=query)
         direct += (not fb) and close(r.pose_vector, ref[k])
         learn_t += time.perf_counter() - t
         learn_ok += close(r.pose_vector, ref[k])

@@ -765,7 +765,8 @@ int ndt_hip_bev_device(const float* d_xyz, const uint32_t* d_offsets, const uint
   hipLaunchKernelGGL(bev_kernel, dim3((unsigned)rows), dim3(256), lds, nullptr, d_xyz, d_offsets, d_index,
                      reinterpret_cast<const float2*>(d_rot), d_bev, nx, ny, nz, cell, zmin, zcell);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipDeviceSynchronize();
+  // The null stream only: other handles' non-blocking streams keep running.
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   return e == hipSuccess ? NDT_OK : NDT_ERR_HIP;
 }
 

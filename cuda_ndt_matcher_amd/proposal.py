@@ -166,6 +166,46 @@ class ScanStore:
         return out
 
 
+class QueryScan:
+    """One scan at a time in device memory for the search: its descriptor in
+    row 0 of a zero-padded 32-row network batch, and the points for
+    align_batch_device.  One pinned upload per scan; buffers are reused."""
+
+    def __init__(self, capacity: int = 1 << 16):
+        import torch
+
+        dev = torch.device("cuda:0")
+        self.batch = torch.zeros((32, int(np.prod(BEV_SHAPE))), dtype=torch.int16, device=dev)
+        self._off_h = torch.zeros(2, dtype=torch.int32).pin_memory()
+        self._off = torch.zeros(2, dtype=torch.int32, device=dev)
+        self._cap = 0
+        self._grow(capacity)
+
+    def _grow(self, n: int) -> None:
+        import torch
+
+        self._cap = max(n, 2 * self._cap)
+        self._h = torch.empty((self._cap, 3), dtype=torch.float32).pin_memory()
+        self.points = torch.empty((self._cap, 3), dtype=torch.float32, device="cuda:0")
+
+    def load(self, scan: np.ndarray):
+        """Uploads `scan` and forms its descriptor; returns (device pointer, n)."""
+        import torch
+
+        a = np.ascontiguousarray(np.asarray(scan, np.float32).reshape(-1, 3))
+        n = a.shape[0]
+        if n > self._cap:
+            self._grow(n)
+        self._h[:n].numpy()[...] = a
+        self.points[:n].copy_(self._h[:n], non_blocking=True)
+        self._off_h[1] = n
+        self._off.copy_(self._off_h, non_blocking=True)
+        # The descriptor kernel runs on another stream than torch's copies.
+        torch.cuda.synchronize()
+        bev_device(self.points.data_ptr(), self._off.data_ptr(), 1, self.batch.data_ptr())
+        return self.points.data_ptr(), n
+
+
 def bev_batch(scans: Sequence[np.ndarray], yaw: Optional[np.ndarray] = None):
     """Descriptors of host scans (rotated by yaw[j] about z when given), as a
     bf16 torch tensor (batch x 32768) on cuda:0."""
@@ -296,7 +336,7 @@ def propose(net: ProposalNet, frame: Frame, bevs, k: int) -> np.ndarray:
 
 def learned_search(matcher, net: ProposalNet, frame: Frame, sample: Sample, bev_row, k: int = HYPOTHESES,
                    nvtl_threshold: float = 2.3, polish: int = 4, fallback=None, times: Optional[dict] = None,
-                   form_descriptor: bool = False):
+                   query: Optional[QueryScan] = None):
     """Proposals -> one batched refinement of the top k.  When the best
     refined pose (by NVTL) has not converged, the `polish` best refined poses
     run once more from where they stopped (one more batched launch: a
@@ -304,9 +344,10 @@ def learned_search(matcher, net: ProposalNet, frame: Frame, sample: Sample, bev_
     before it converges).  The plain search (fallback(sample)) runs when the
     best NVTL is still below the threshold.  Height comes from the prior
     (the sample's pose), roll and pitch start at zero.
-    With form_descriptor the descriptor is formed here from the sample's scan
-    (bev_row is ignored).  `times`, if given, accumulates seconds per stage
-    (descriptor, proposal, refine, polish, fallback).
+    With a QueryScan the scan is uploaded once, its descriptor formed on the
+    GPU (bev_row is ignored) and the refinements read the device copy.
+    `times`, if given, accumulates seconds per stage (descriptor, proposal,
+    refine, polish, fallback).
     Returns (best AlignResult, used_fallback)."""
     import time
 
@@ -316,17 +357,28 @@ def learned_search(matcher, net: ProposalNet, frame: Frame, sample: Sample, bev_
         return time.perf_counter()
 
     t = time.perf_counter()
-    if form_descriptor:
-        bev_row = bev_batch([sample.scan])
+    if query is not None:
+        ptr, n = query.load(sample.scan)
         t = lap("descriptor", t)
-    hyp = propose(net, frame, bev_row, k)[0]
+        out = net.forward_device(query.batch.data_ptr(), query.batch.shape[0])[:1]
+        h = frame.decode(out)
+        order = np.argsort(-h[..., 3], axis=1)[:, :k]
+        hyp = np.take_along_axis(h[..., :3], order[..., None], axis=1)[0]
+
+        def refine(poses):
+            return matcher.align_batch_device(ptr, n, poses)
+    else:
+        hyp = propose(net, frame, bev_row, k)[0]
+
+        def refine(poses):
+            return matcher.align_batch(sample.scan, poses)
     t = lap("proposal", t)
     z = sample.true_pose[2]
     poses = [np.array([h[0], h[1], z, 0.0, 0.0, h[2]]) for h in hyp]
-    res = sorted(matcher.align_batch(sample.scan, poses), key=lambda r: -r.nvtl)
+    res = sorted(refine(poses), key=lambda r: -r.nvtl)
     t = lap("refine", t)
     if polish > 0 and not res[0].converged:
-        again = matcher.align_batch(sample.scan, [r.pose_vector for r in res[:polish]])
+        again = refine([r.pose_vector for r in res[:polish]])
         res = sorted(again + res, key=lambda r: -r.nvtl)
         t = lap("polish", t)
     best = res[0]

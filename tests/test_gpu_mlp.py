@@ -96,3 +96,30 @@ def test_training_reduces_the_proposal_loss():
     assert after < 0.35 * before, (before, after)
     ref, _ = R.wta_loss(out, labels)
     assert abs(ref - net.train_step_device(xb.data_ptr(), labels, 0.0)) <= 1e-3 * max(1.0, abs(ref))
+
+
+def test_query_scan_search_matches_the_batch_path():
+    """QueryScan: the descriptor row equals bev_batch's, the padding rows stay
+    zero, and a learned search through the device copy of the scan returns the
+    same alignment as through the host scan."""
+    import torch
+
+    from cuda_ndt_matcher_amd import NdtScanMatcher, proposal, synth
+
+    _print_case("query scan")
+    wl = synth.make_workload("small", 0)
+    m = NdtScanMatcher.builder().resolution(2.0).max_iterations(30).transformation_epsilon(0.01).build()
+    m.set_target(wl.map_points)
+    samples = proposal.make_samples(0, 5, 2, 40.0)
+    net = proposal.ProposalNet(proposal.NET_SIZES, seed=1)
+    frame = proposal.Frame(0.0, 0.0, 4.0)
+    q = proposal.QueryScan(capacity=64)  # grows on the first scan
+    for smp in samples:
+        q.load(smp.scan)
+        want = proposal.bev_batch([smp.scan])
+        torch.cuda.synchronize()
+        assert torch.equal(q.batch[:1], want)
+        assert not torch.any(q.batch[1:])
+        a, fa = proposal.learned_search(m, net, frame, smp, None, query=q)
+        b, fb = proposal.learned_search(m, net, frame, smp, want)
+        assert fa == fb and np.array_equal(a.pose_vector, b.pose_vector) and a.iterations == b.iterations
