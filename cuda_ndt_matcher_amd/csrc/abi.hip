@@ -83,6 +83,20 @@ int hip_err(ndt_handle* h, hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_err(h, e_, what); \
   } while (0)
 
+// A cooperative launch (several workgroups per alignment) fails with
+// hipErrorIllegalState while another process on the device has made
+// cooperative launches: the runtime serves them to one process at a time, and
+// a process keeps the resource until it exits.
+int launch_err(ndt_handle* h, hipError_t e, const char* what, bool coop) {
+  if (coop && e == hipErrorIllegalState)
+    return set_err(h, NDT_ERR_HIP,
+                   std::string(what) +
+                       ": cooperative launch refused: another process on this GPU has made cooperative launches "
+                       "(the runtime serves them to one process at a time, until that process exits); "
+                       "workgroups_per_alignment = 1 needs no cooperative launch");
+  return hip_err(h, e, what);
+}
+
 void gauss_params(double res, double outlier, double* d1, double* d2) {
   // GaussianParams::new (derivatives/types.rs:61-75)
   const double c1 = 10.0 * (1.0 - outlier);
@@ -203,15 +217,18 @@ int launch_once(ndt_handle* h, size_t m, int W, bool allow_graph, float* ms, boo
       std::memcpy(&c.cfg, &cfg, sizeof(cfg));
       std::memcpy(&c.view, &h->grid.view, sizeof(c.view));
     }
-    NDT_HIP(hipGraphLaunch(c.exec, h->stream), "graph launch");
+    if (hipError_t e = hipGraphLaunch(c.exec, h->stream); e != hipSuccess)
+      return launch_err(h, e, "graph launch", W > 1);
   } else {
     if (h->cfg.profile_phases)
       NDT_HIP(hipMemsetAsync(h->phase.ptr, 0, 16 * sizeof(unsigned long long), h->stream), "clear phase");
     NDT_HIP(hipMemcpyAsync(h->jobs.ptr, h->host_jobs, m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
             "copy jobs");
-    NDT_HIP(ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr), (int)m,
-                               reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream, W > 1, h->ev0),
-            "newton launch");
+    if (hipError_t e = ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr),
+                                          (int)m, reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream,
+                                          W > 1, h->ev0);
+        e != hipSuccess)
+      return launch_err(h, e, "newton launch", W > 1);
     NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
     NDT_HIP(hipMemcpyAsync(h->host_results, h->results.ptr, m * sizeof(ndt_result), hipMemcpyDeviceToHost,
                            h->stream),

@@ -98,7 +98,10 @@ int main(void) {
 
   const double guess[6] = {1.8, -1.8, 1.2, 0.0, 0.0, 0.35 + 2.0 * M_PI / 180.0};
   ndt_result r;
-  check(ndt_hip_align(h, scan, m, guess, &r) == NDT_OK, "align");
+  memset(&r, 0, sizeof(r));
+  const int arc = ndt_hip_align(h, scan, m, guess, &r);
+  if (arc != NDT_OK) printf("     ndt_hip_align returned %d: %s\n", arc, ndt_hip_last_error(h));
+  check(arc == NDT_OK, "align");
   double et = 0.0, er = 0.0;
   for (int i = 0; i < 3; ++i) et = fmax(et, fabs(r.pose[i] - truth[i]));
   for (int i = 3; i < 6; ++i) er = fmax(er, fabs(r.pose[i] - truth[i]));

@@ -1,6 +1,11 @@
 """The library from plain C (GPU): a gcc-built program that includes only
 include/ndt_hip.h and links -lndt_hip aligns a scan of a known pose, checks
-batch against single alignments, NVTL evaluation and an error path."""
+batch against single alignments, NVTL evaluation and an error path.
+
+The program is a second process on the GPU.  Its alignments use cooperative
+launches, which the runtime serves to one process at a time: once this test
+process has made one (any other GPU test does), the child's are refused until
+this process exits.  The file name makes this test the first one collected."""
 import os
 import subprocess
 import sys
