@@ -365,6 +365,57 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     return out
 
 
+def scoring_bench(m, wl, ptr, n, seed, with_cpu=True, reps=10):
+    """Batched scoring on the 30k-point scan and the 5M-point map (device
+    resident scan): evaluate_nvtl_batch over 200 poses around the true pose
+    (the 3x3x3-cell NVTL of scoring/nvtl.rs:83-190), evaluate_nvtl and
+    evaluate_transform_probability at one pose, per-point scores.  Device time
+    from the library's events (score_ms) and host wall time per call; the
+    oracle's single-core time for the NVTL batch on a sample of the poses."""
+    rng = np.random.default_rng(seed + 11)
+    poses = [wl.true_pose + np.array([rng.normal(0, 0.5), rng.normal(0, 0.5), 0.0, 0.0, 0.0, rng.normal(0, 0.05)])
+             for _ in range(200)]
+    out = {}
+
+    def timed(label, fn, units):
+        fn()
+        t0 = time.perf_counter()
+        dev = []
+        for _ in range(reps):
+            fn()
+            dev.append(m.timing()["score_ms"])
+        wall = (time.perf_counter() - t0) * 1e3 / reps
+        d = float(np.median(dev))
+        out[label] = {"wall_ms": wall, "kernel_ms": d, "point_evals_per_s": units / (d * 1e-3)}
+
+    scan = wl.scan
+    timed("nvtl_batch_200_poses", lambda: m.evaluate_nvtl_batch(scan, poses), 200 * n)
+    timed("nvtl", lambda: m.evaluate_nvtl(scan, wl.true_pose), n)
+    timed("transform_probability", lambda: m.evaluate_transform_probability(scan, wl.true_pose), n)
+    timed("per_point_scores", lambda: m.compute_per_point_scores(scan, wl.true_pose), n)
+    if with_cpu:
+        from oracle import oracle as O
+
+        g = O.Grid(wl.map_points, 2.0)
+        p = O.params()
+        t0 = time.perf_counter()
+        k = 0
+        while k < len(poses) and (k < 2 or time.perf_counter() - t0 < 5.0):
+            g.nvtl_cell27(scan, poses[k], p)
+            k += 1
+        per_pose = (time.perf_counter() - t0) * 1e3 / k
+        out["nvtl_batch_200_poses"]["cpu_baseline"] = {
+            "value": per_pose * 200, "unit": "ms per 200-pose batch", "cores": 1, "kind": "port",
+            "sample": f"{k} poses of the batch on one core (oracle nvtl_cell27), scaled to 200"}
+    b = out["nvtl_batch_200_poses"]
+    print(f"[bench] scoring: NVTL of 200 poses {b['kernel_ms']:.3f} ms kernel ({b['wall_ms']:.3f} ms wall, "
+          f"{b['point_evals_per_s'] / 1e9:.2f} G point-evals/s); NVTL {out['nvtl']['kernel_ms']:.3f} ms, TP "
+          f"{out['transform_probability']['kernel_ms']:.3f} ms, per-point {out['per_point_scores']['kernel_ms']:.3f} ms"
+          + (f"; oracle {b['cpu_baseline']['value']:.0f} ms per batch" if with_cpu else ""), file=sys.stderr,
+          flush=True)
+    return out
+
+
 def ingest_bench(seed, with_cpu=True, reps=20):
     """Either side of the alignment.  (1) The sensor point filter on the
     128-beam scan of the 5M-point workload (about 230k points, device
@@ -521,6 +572,8 @@ def main():
     ap.add_argument("--learned-epochs", type=int, default=60, help="training epochs for --learned")
     ap.add_argument("--serve", action="store_true",
                     help="also serve a Poisson stream of 1k-40k-point requests (continuous batching)")
+    ap.add_argument("--scoring", action="store_true",
+                    help="also time batched NVTL, TP and per-point scoring on the 30k-point scan")
     ap.add_argument("--ingest", action="store_true",
                     help="also time the sensor point filter and alignments during a background map rebuild")
     ap.add_argument("--batching", action="store_true",
@@ -656,6 +709,8 @@ def main():
         extra["batching"] = batching_bench(wl, args.seed)
     if args.ingest:
         extra["ingest"] = ingest_bench(args.seed, with_cpu=not args.no_cpu)
+    if args.scoring:
+        extra["scoring"] = scoring_bench(m, wl, ptr, n, args.seed, with_cpu=not args.no_cpu)
     if args.precision_sweep:
         extra["precision_sweep"] = precision_sweep(args.seed, max(3, args.steps // 5))
     line.update(extra)
