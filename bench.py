@@ -39,10 +39,12 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
-def newton_flops(n_points, corr, iterations):
-    """Algorithmic FLOP of one alignment: per iteration 111 per point + 330 per
-    point-voxel pair, plus the final NVTL pass (about 40 per pair)."""
-    return iterations * (111.0 * n_points + 330.0 * corr) + 40.0 * corr
+def newton_flops(n_points, pair_evaluations, iterations, final_corr):
+    """Algorithmic FLOP of one alignment: 111 per point per iteration, 330 per
+    point-voxel pair evaluated in the derivative passes (summed over the
+    iterations, as the kernel reports it), plus the final NVTL pass (about 40
+    per pair)."""
+    return iterations * 111.0 * n_points + 330.0 * pair_evaluations + 40.0 * final_corr
 
 
 def build_bytes(n, voxels, key_bits):
@@ -631,7 +633,7 @@ def main():
     ms_per_step = 1e3 * el / args.steps
     value = args.steps / el
     k_ms = float(np.mean(kernel_ms))
-    flops = newton_flops(n, r.num_correspondences, r.iterations)
+    flops = newton_flops(n, r.pair_evaluations, r.iterations, r.num_correspondences)
     achieved = flops / (k_ms * 1e-3) / 1e12
     print(f"[bench] {value:.1f} aligns/s, kernel {k_ms:.3f} ms, iterations {r.iterations}, "
           f"corr {r.num_correspondences}, status {r.status}", file=sys.stderr, flush=True)

@@ -59,7 +59,7 @@ class NdtResult(C.Structure):
         ("iterations", C.c_uint32),
         ("num_correspondences", C.c_uint32),
         ("oscillation_count", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("pair_evaluations", C.c_uint32),
         ("hessian", C.c_double * 36),
     ]
 

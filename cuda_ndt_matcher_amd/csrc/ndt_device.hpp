@@ -355,30 +355,23 @@ struct PoseTables {
   double h[15][3];  // angular Hessian (angular.rs:104-181)
 };
 
-__device__ inline void snap_sincos(double a, double* s, double* c) {
-  if (fabs(a) < 1e-5) {
-    *s = 0.0;
-    *c = 1.0;
-  } else {
-    sincos(a, s, c);
-  }
-}
-
-__device__ inline void make_pose_tables(const double* pose, PoseTables* T) {
+// The tables from the pose and the exact sin/cos of its angles, trig =
+// {sin r, cos r, sin p, cos p, sin y, cos y}: the rotation from the exact
+// values, the angular terms from the values with the snap of
+// angular.rs:39-55 (|angle| < 1e-5: sin 0, cos 1).
+__device__ inline void make_pose_tables_trig(const double* pose, const double* trig, PoseTables* T) {
   T->t[0] = pose[0];
   T->t[1] = pose[1];
   T->t[2] = pose[2];
-  double sr, cr, sp, cp, sy, cy;
-  sincos(pose[3], &sr, &cr);
-  sincos(pose[4], &sp, &cp);
-  sincos(pose[5], &sy, &cy);
+  double sr = trig[0], cr = trig[1], sp = trig[2], cp = trig[3], sy = trig[4], cy = trig[5];
   T->R[0] = cp * cy;                T->R[1] = -cp * sy;               T->R[2] = sp;
   T->R[3] = sr * sp * cy + cr * sy; T->R[4] = cr * cy - sr * sp * sy; T->R[5] = -sr * cp;
   T->R[6] = sr * sy - cr * sp * cy; T->R[7] = cr * sp * sy + sr * cy; T->R[8] = cr * cp;
-  double sx, cx, sz, cz;
-  snap_sincos(pose[3], &sx, &cx);
-  snap_sincos(pose[4], &sy, &cy);
-  snap_sincos(pose[5], &sz, &cz);
+  const bool snx = fabs(pose[3]) < 1e-5, sny = fabs(pose[4]) < 1e-5, snz = fabs(pose[5]) < 1e-5;
+  const double sx = snx ? 0.0 : trig[0], cx = snx ? 1.0 : trig[1];
+  sy = sny ? 0.0 : trig[2];
+  cy = sny ? 1.0 : trig[3];
+  const double sz = snz ? 0.0 : trig[4], cz = snz ? 1.0 : trig[5];
   auto& J = T->j;
   J[0][0] = -sx * sz + cx * sy * cz; J[0][1] = -sx * cz - cx * sy * sz; J[0][2] = -cx * cy;
   J[1][0] = cx * sz + sx * sy * cz;  J[1][1] = cx * cz - sx * sy * sz;  J[1][2] = -sx * cy;
@@ -404,6 +397,15 @@ __device__ inline void make_pose_tables(const double* pose, PoseTables* T) {
   H[12][0] = -cy * cz;                H[12][1] = cy * sz;                 H[12][2] = 0.0;
   H[13][0] = -cx * sz - sx * sy * cz; H[13][1] = -cx * cz + sx * sy * sz; H[13][2] = 0.0;
   H[14][0] = -sx * sz + cx * sy * cz; H[14][1] = -cx * sy * sz - sx * cz; H[14][2] = 0.0;
+}
+
+// One thread: the three sin/cos pairs, then the tables.
+__device__ inline void make_pose_tables(const double* pose, PoseTables* T) {
+  double trig[6];
+  sincos(pose[3], &trig[0], &trig[1]);
+  sincos(pose[4], &trig[2], &trig[3]);
+  sincos(pose[5], &trig[4], &trig[5]);
+  make_pose_tables_trig(pose, trig, T);
 }
 
 // nalgebra UnitQuaternion::from_euler_angles (w, i, j, k), used by the
@@ -623,6 +625,23 @@ constexpr int kJacobiPairs[5][3][2] = {{{0, 1}, {2, 3}, {4, 5}}, {{0, 2}, {1, 4}
 
 // Orders LDS accesses of the lanes of one wave (no cross-wave barrier).
 __device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
+
+// make_pose_tables by all lanes of one wave: lanes 0-2 evaluate the three
+// sin/cos pairs at once (each is a long dependent chain), lane 0 forms the
+// tables.  pose (6) and trig (6) are in LDS; pose must be written before the
+// call by a lane of the same wave.
+__device__ inline void make_pose_tables_wave(const double* pose, double* trig, PoseTables* T) {
+  const int lane = threadIdx.x & (kWave - 1);
+  wave_lds_fence();
+  if (lane < 3) {
+    double sn, cs;
+    sincos(pose[3 + lane], &sn, &cs);
+    trig[2 * lane] = sn;
+    trig[2 * lane + 1] = cs;
+  }
+  wave_lds_fence();
+  if (lane == 0) make_pose_tables_trig(pose, trig, T);
+}
 
 // Value of x in lane l (l uniform), as a scalar.
 __device__ inline double readlane_f64(double x, int l) {

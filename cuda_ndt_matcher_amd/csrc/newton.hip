@@ -447,6 +447,7 @@ struct SlotState {
   int ctrl;         // 0: take a step, 1: stop, 2: line search, 3: break to the max-iterations result
   int need_solve;
   uint32_t last_corr, final_corr;
+  uint32_t corr_total;  // pairs summed over the derivative passes with the Hessian
 };
 
 // Slot-wide exchange state, identical in every workgroup of a slot.
@@ -543,25 +544,39 @@ __device__ inline void slot_reduce(double (&v)[N], int count, double* lds, doubl
   __syncthreads();
 }
 
-__device__ void regularization_terms(const NewtonConfig& cfg, const double* pose, uint32_t corr, double* rs,
-                                     double rg[6], double rH[36]) {
-  // RegularizationTerm::compute_derivatives (optimization/regularization.rs:102-159)
-  *rs = 0.0;
-  for (int i = 0; i < 6; ++i) rg[i] = 0.0;
-  for (int i = 0; i < 36; ++i) rH[i] = 0.0;
-  if (!cfg.reg_active) return;
+// RegularizationTerm::compute_derivatives (optimization/regularization.rs:102-159):
+// score, gradient (x, y) and Hessian (xy block); zero when inactive.
+struct RegTerms {
+  double s, g0, g1, h00, h01, h11;
+};
+__device__ inline RegTerms regularization_xy(const NewtonConfig& cfg, const double* pose, uint32_t corr) {
+  RegTerms r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (!cfg.reg_active) return r;
   const double scale = cfg.reg_scale, w = (double)corr;
   const double dx = cfg.reg_t[0] - pose[0], dy = cfg.reg_t[1] - pose[1];
   double sy, cy;
   sincos(pose[5], &sy, &cy);
   const double lon = dy * sy + dx * cy;
-  *rs = -scale * w * lon * lon;
-  rg[0] = scale * w * 2.0 * cy * lon;
-  rg[1] = scale * w * 2.0 * sy * lon;
-  rH[0] = -scale * w * 2.0 * cy * cy;
-  rH[1] = -scale * w * 2.0 * cy * sy;
-  rH[6] = rH[1];
-  rH[7] = -scale * w * 2.0 * sy * sy;
+  r.s = -scale * w * lon * lon;
+  r.g0 = scale * w * 2.0 * cy * lon;
+  r.g1 = scale * w * 2.0 * sy * lon;
+  r.h00 = -scale * w * 2.0 * cy * cy;
+  r.h01 = -scale * w * 2.0 * cy * sy;
+  r.h11 = -scale * w * 2.0 * sy * sy;
+  return r;
+}
+
+// Element (i, j) of the regularisation Hessian.
+__device__ inline double reg_hessian(const RegTerms& r, int i, int j) {
+  return (i > 1 || j > 1) ? 0.0 : (i == 0 && j == 0 ? r.h00 : (i == 1 && j == 1 ? r.h11 : r.h01));
+}
+
+__device__ void regularization_terms(const NewtonConfig& cfg, const double* pose, uint32_t corr, double* rs,
+                                     double rg[6], double rH[36]) {
+  const RegTerms r = regularization_xy(cfg, pose, corr);
+  *rs = r.s;
+  for (int i = 0; i < 6; ++i) rg[i] = i == 0 ? r.g0 : (i == 1 ? r.g1 : 0.0);
+  for (int i = 0; i < 36; ++i) rH[i] = reg_hessian(r, i / 6, i % 6);
 }
 
 __device__ uint32_t oscillation_count(const double* h, int k) {
@@ -634,6 +649,7 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
   double* const s_red = s_big.red;
   const Stage stage{s_big.rec, s_stage_slot};
   __shared__ double s_q[4];
+  __shared__ double s_tp[6], s_trig[6];  // trial pose; sin/cos of the tables' angles
 
   const int W = sy.W;
   const int rank = sy.rank;
@@ -668,12 +684,14 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
     S.nhist = 1;
     S.status = -1;
     S.last_corr = 0;
+    S.corr_total = 0;
     s_solve.have_basis = 0;
     s_solve.sweeps = 0;
     s_solve.solves = 0;
     newton_step_reset(s_solve);
-    make_pose_tables(S.pose, &s_tab);
   }
+  // Pose tables: wave 0 (three lanes evaluate the angles' sin/cos at once).
+  if (threadIdx.x < kWave) make_pose_tables_wave(S.pose, s_trig, &s_tab);
   __syncthreads();
 
   for (int it = 0; it < max_iter; ++it) {
@@ -697,6 +715,7 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
 
     if (threadIdx.x == 0) {
       const uint32_t c = (uint32_t)s_tot[kAccVals];
+      S.corr_total += c;
       S.ctrl = 0;
       S.need_solve = 0;
       if (c < 10) {  // solver.rs:168-174
@@ -708,16 +727,7 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
         }
       } else {
         S.last_corr = c;
-        double rs, rg[6], rH[36];
-        regularization_terms(cfg, S.pose, c, &rs, rg, rH);
-        S.score = s_tot[0] + rs;
-        for (int i = 0; i < 6; ++i) S.grad[i] = s_tot[1 + i] + rg[i];
-        for (int i = 0; i < 6; ++i)
-          for (int j = 0; j <= i; ++j) {
-            const double v = s_tot[7 + tri(i, j)];
-            S.last_H[i * 6 + j] = v + rH[i * 6 + j];
-            S.last_H[j * 6 + i] = v + rH[j * 6 + i];
-          }
+        S.score = s_tot[0] + regularization_xy(cfg, S.pose, c).s;
         if (S.score > S.best_score) {
           S.best_score = S.score;
           for (int i = 0; i < 6; ++i) S.best_pose[i] = S.pose[i];
@@ -725,6 +735,17 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
         S.need_solve = 1;
         newton_step_reset(s_solve);
       }
+    }
+    // Gradient and full Hessian (plus the regularisation terms, which touch
+    // only g0, g1 and the xy block) written by the lanes of wave 0 at once.
+    if (threadIdx.x < kWave && (uint32_t)s_tot[kAccVals] >= 10) {
+      const int l = threadIdx.x;
+      const RegTerms r = regularization_xy(cfg, S.pose, (uint32_t)s_tot[kAccVals]);
+      if (l < 36) {
+        const int i = l / 6, j = l % 6;
+        S.last_H[l] = s_tot[7 + (i >= j ? tri(i, j) : tri(j, i))] + reg_hessian(r, i, j);
+      }
+      if (l < 6) S.grad[l] = s_tot[1 + l] + (l == 0 ? r.g0 : (l == 1 ? r.g1 : 0.0));
     }
     __syncthreads();
     if (rec) t_solve0 = wall_clock64();
@@ -778,11 +799,12 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
           }
         }
       }
-      if (S.ctrl == 2) {
-        double tp[6];
-        for (int i = 0; i < 6; ++i) tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
-        make_pose_tables(tp, &s_tab);
-      }
+      if (S.ctrl == 2)
+        for (int i = 0; i < 6; ++i) s_tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
+    }
+    if (threadIdx.x < kWave) {
+      wave_lds_fence();
+      if (S.ctrl == 2) make_pose_tables_wave(s_tp, s_trig, &s_tab);
     }
     __syncthreads();
     if (rec) ph[2] += wall_clock64() - t_red;
@@ -812,9 +834,12 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
           if (S.mt.done) {
             S.step = S.mt.conv ? S.mt.a_t : kMtMax;
           } else {
-            for (int i = 0; i < 6; ++i) tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
-            make_pose_tables(tp, &s_tab);
+            for (int i = 0; i < 6; ++i) s_tp[i] = S.pose[i] + S.dir[i] * S.mt.a_t;
           }
+        }
+        if (threadIdx.x < kWave) {
+          wave_lds_fence();
+          if (!S.mt.done) make_pose_tables_wave(s_tp, s_trig, &s_tab);
         }
         __syncthreads();
         if (rec) ++n_ls;
@@ -828,8 +853,8 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
       S.hist[3 * S.nhist + 1] = S.pose[1];
       S.hist[3 * S.nhist + 2] = S.pose[2];
       S.nhist += 1;
-      make_pose_tables(S.pose, &s_tab);
     }
+    if (threadIdx.x < kWave) make_pose_tables_wave(S.pose, s_trig, &s_tab);
     __syncthreads();
   }
 
@@ -889,7 +914,7 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
       r.oscillation_count = oscillation_count(S.hist, S.nhist);
       for (int i = 0; i < 36; ++i) r.hessian[i] = S.final_H[i];
     }
-    r.reserved = 0;
+    r.pair_evaluations = S.corr_total;
     *out = r;
   }
 }

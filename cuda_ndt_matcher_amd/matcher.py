@@ -37,10 +37,10 @@ class AlignResult:
     Isometry3) is formed from `pose_vector` when first read."""
 
     __slots__ = ("pose_vector", "converged", "status", "score", "transform_probability", "nvtl", "iterations",
-                 "hessian", "num_correspondences", "oscillation_count", "_pose")
+                 "hessian", "num_correspondences", "oscillation_count", "pair_evaluations", "_pose")
 
     def __init__(self, pose_vector, converged, status, score, transform_probability, nvtl, iterations, hessian,
-                 num_correspondences, oscillation_count, pose=None):
+                 num_correspondences, oscillation_count, pose=None, pair_evaluations=0):
         self.pose_vector = pose_vector
         self.converged = converged
         self.status = status
@@ -51,6 +51,7 @@ class AlignResult:
         self.hessian = hessian
         self.num_correspondences = num_correspondences
         self.oscillation_count = oscillation_count
+        self.pair_evaluations = pair_evaluations  # pairs over the Hessian passes (work done; not in the reference)
         self._pose = pose
 
     @property
@@ -211,8 +212,9 @@ def _results(arr, m: int) -> List[AlignResult]:
     status, conv = a["status"].tolist(), a["converged"].tolist()
     score, tp, nvtl = a["score"].tolist(), a["transform_probability"].tolist(), a["nvtl"].tolist()
     its, corr, osc = a["iterations"].tolist(), a["num_correspondences"].tolist(), a["oscillation_count"].tolist()
+    pe = a["pair_evaluations"].tolist()
     return [AlignResult(poses[i], bool(conv[i]), STATUS_NAMES.get(status[i], str(status[i])), score[i], tp[i], nvtl[i],
-                        its[i], hess[i], corr[i], osc[i]) for i in range(m)]
+                        its[i], hess[i], corr[i], osc[i], pair_evaluations=pe[i]) for i in range(m)]
 
 
 def _result(r: _lib.NdtResult) -> AlignResult:

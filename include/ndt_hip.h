@@ -117,7 +117,8 @@ typedef struct ndt_result {
   uint32_t iterations;
   uint32_t num_correspondences;
   uint32_t oscillation_count;
-  uint32_t reserved;
+  uint32_t pair_evaluations;     /* point-voxel pairs summed over the derivative passes with the Hessian
+                                    (work done; not part of the reference's result) */
   double hessian[36];            /* row-major */
 } ndt_result;
 

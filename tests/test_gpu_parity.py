@@ -383,3 +383,23 @@ def test_full_size_parity(oracle_mod):
     o, ok = oracle_stable(og, w.scan, start, p)
     assert ok
     _compare_align(m.align(w.scan, start), o, w.scan.shape[0])
+
+
+def test_pair_evaluations_count_the_derivative_passes(small):
+    """pair_evaluations sums the point-voxel pairs of the alignment's
+    derivative passes: with one iteration it is the pass at the initial pose;
+    over a whole alignment it is at least the final pass's count and, for a
+    start near the optimum, close to iterations times it."""
+    from cuda_ndt_matcher_amd import NdtScanMatcher
+
+    _print_case("pair evaluations")
+    one = NdtScanMatcher.builder().resolution(2.0).max_iterations(1).search_mode(2).build()
+    one.set_target(small.map_points)
+    _, _, _, c0 = one.derivatives(small.scan, small.guess)
+    r1 = one.align(small.scan, small.guess)
+    assert r1.iterations == 1 and r1.pair_evaluations == c0 == r1.num_correspondences
+    full = NdtScanMatcher.builder().resolution(2.0).max_iterations(30).search_mode(2).build()
+    full.set_target(small.map_points)
+    r = full.align(small.scan, small.true_pose)
+    assert r.pair_evaluations >= r.num_correspondences
+    assert abs(r.pair_evaluations - r.iterations * r.num_correspondences) <= 0.05 * r.pair_evaluations
