@@ -505,10 +505,10 @@ def ingest_bench(seed, with_cpu=True, reps=20):
     return out
 
 
-def cpu_baseline(wl, budget_s=12.0):
+def cpu_baseline(wl, budget_s=12.0, grid=None):
     from oracle import oracle as O
 
-    g = O.Grid(wl.map_points, 2.0)
+    g = grid if grid is not None else O.Grid(wl.map_points, 2.0)
     p = O.params()
     t0 = time.perf_counter()
     k = 0
@@ -521,6 +521,39 @@ def cpu_baseline(wl, budget_s=12.0):
     return {"value": k / el, "unit": "aligns/s", "cores": 1, "kind": "port",
             "sample": f"{k} single-core aligns of the same {wl.scan.shape[0]}-pt scan against the same 5M-pt map "
                       f"(C++ FP64 restatement of the reference CPU path, -O2), {el:.1f} s"}
+
+
+def parity_line(m, wl, grid):
+    """The oracle beside the GPU on the full-size workload.  A well-conditioned
+    start (0.4 m, 1.5 deg from the true pose): pose, rotation, iteration and
+    NVTL differences.  The bench's own start: how far the oracle's answer moves
+    when every iteration's sums are perturbed at 1e-12 (three seeds), against
+    the GPU's distance from it; that start wanders at max iterations, so the
+    two are of the same size."""
+    from oracle import oracle as O
+
+    p = O.params()
+    rng = np.random.default_rng(5)
+    start = wl.true_pose + np.array([rng.normal(0, 0.4), rng.normal(0, 0.4), 0.0, 0.0, 0.0, np.deg2rad(1.5)])
+    o = grid.align(wl.scan, start, p)
+    r = m.align(wl.scan, start)
+    rot = np.rad2deg(np.abs(np.angle(np.exp(1j * (np.asarray(r.pose_vector[3:]) - np.asarray(o.pose[3:]))))).max())
+    well = {"pose_err_mm": 1e3 * float(np.abs(np.asarray(r.pose_vector[:3]) - np.asarray(o.pose[:3])).max()),
+            "rot_err_deg": float(rot), "iterations": [int(r.iterations), int(o.iterations)],
+            "nvtl_abs_err": abs(r.nvtl - o.nvtl), "tp_abs_err": abs(r.transform_probability - o.transform_probability),
+            "status": [r.status, ["Converged", "MaxIterations", "NoCorrespondences", "SingularHessian"][o.status]]}
+    ob = grid.align(wl.scan, wl.guess, p)
+    spread = 0.0
+    for seed in (1, 2, 3):
+        q = grid.align(wl.scan, wl.guess, O.params(noise=1e-12, noise_seed=seed))
+        spread = max(spread, float(np.abs(np.asarray(q.pose[:3]) - np.asarray(ob.pose[:3])).max()))
+    rb = m.align(wl.scan, wl.guess)
+    bench = {"oracle_self_spread_m": spread,
+             "gpu_vs_oracle_m": float(np.abs(np.asarray(rb.pose_vector[:3]) - np.asarray(ob.pose[:3])).max())}
+    print(f"[bench] parity: well-conditioned start {well['pose_err_mm']:.4f} mm {well['rot_err_deg']:.5f} deg, "
+          f"iterations {well['iterations']}; bench start: oracle moves {spread:.3f} m under 1e-12 perturbations, "
+          f"GPU {bench['gpu_vs_oracle_m']:.3f} m from it", file=sys.stderr, flush=True)
+    return {"well_conditioned_start": well, "bench_start": bench}
 
 
 def cpu_particle_baseline(map_points, cases, budget_s=12.0, threads=None):
@@ -717,7 +750,11 @@ def main():
         extra["precision_sweep"] = precision_sweep(args.seed, max(3, args.steps // 5))
     line.update(extra)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(wl)
+        from oracle import oracle as O
+
+        grid = O.Grid(wl.map_points, 2.0)
+        line["cpu_baseline"] = cpu_baseline(wl, grid=grid)
+        line["parity"] = parity_line(m, wl, grid)
     print(json.dumps(line), flush=True)
 
 
