@@ -41,6 +41,10 @@ KAT_CASES = [
     ([[-0.1, -0.1, -0.1], [-0.2, -0.2, -0.2]], dict(downsample_resolution=1.0), None, [2, 0, 0, 1]),
     ([[1, 2, 3]], dict(downsample_resolution=0.0), [[1, 2, 3]], [1, 0, 0, 0]),
     ([[1, 2, 3]], dict(downsample_resolution=-1.0), [[1, 2, 3]], [1, 0, 0, 0]),
+    # a NaN or infinite resolution is not <= 0, so the cells are floor(p * NaN)
+    # or floor(p * 0) = 0 after the saturating cast: one cell for everything
+    ([[1, 2, 3], [-7, 5, 0.5]], dict(downsample_resolution=float("nan")), [[-3, 3.5, 1.75]], [2, 0, 0, 1]),
+    ([[1, 2, 3], [-7, 5, 0.5]], dict(downsample_resolution=float("inf")), [[-3, 3.5, 1.75]], [2, 0, 0, 1]),
 ]
 
 
