@@ -355,58 +355,100 @@ struct PoseTables {
   double h[15][3];  // angular Hessian (angular.rs:104-181)
 };
 
-// The tables from the pose and the exact sin/cos of its angles, trig =
-// {sin r, cos r, sin p, cos p, sin y, cos y}: the rotation from the exact
-// values, the angular terms from the values with the snap of
-// angular.rs:39-55 (|angle| < 1e-5: sin 0, cos 1).
-__device__ inline void make_pose_tables_trig(const double* pose, const double* trig, PoseTables* T) {
-  T->t[0] = pose[0];
-  T->t[1] = pose[1];
-  T->t[2] = pose[2];
-  double sr = trig[0], cr = trig[1], sp = trig[2], cp = trig[3], sy = trig[4], cy = trig[5];
-  T->R[0] = cp * cy;                T->R[1] = -cp * sy;               T->R[2] = sp;
-  T->R[3] = sr * sp * cy + cr * sy; T->R[4] = cr * cy - sr * sp * sy; T->R[5] = -sr * cp;
-  T->R[6] = sr * sy - cr * sp * cy; T->R[7] = cr * sp * sy + sr * cy; T->R[8] = cr * cp;
-  const bool snx = fabs(pose[3]) < 1e-5, sny = fabs(pose[4]) < 1e-5, snz = fabs(pose[5]) < 1e-5;
-  const double sx = snx ? 0.0 : trig[0], cx = snx ? 1.0 : trig[1];
-  sy = sny ? 0.0 : trig[2];
-  cy = sny ? 1.0 : trig[3];
-  const double sz = snz ? 0.0 : trig[4], cz = snz ? 1.0 : trig[5];
-  auto& J = T->j;
-  J[0][0] = -sx * sz + cx * sy * cz; J[0][1] = -sx * cz - cx * sy * sz; J[0][2] = -cx * cy;
-  J[1][0] = cx * sz + sx * sy * cz;  J[1][1] = cx * cz - sx * sy * sz;  J[1][2] = -sx * cy;
-  J[2][0] = -sy * cz;                J[2][1] = sy * sz;                 J[2][2] = cy;
-  J[3][0] = sx * cy * cz;            J[3][1] = -sx * cy * sz;           J[3][2] = sx * sy;
-  J[4][0] = -cx * cy * cz;           J[4][1] = cx * cy * sz;            J[4][2] = -cx * sy;
-  J[5][0] = -cy * sz;                J[5][1] = -cy * cz;                J[5][2] = 0.0;
-  J[6][0] = cx * cz - sx * sy * sz;  J[6][1] = -cx * sz - sx * sy * cz; J[6][2] = 0.0;
-  J[7][0] = sx * cz + cx * sy * sz;  J[7][1] = cx * sy * cz - sx * sz;  J[7][2] = 0.0;
-  auto& H = T->h;
-  H[0][0] = -cx * sz - sx * sy * cz;  H[0][1] = -cx * cz + sx * sy * sz;  H[0][2] = sx * cy;
-  H[1][0] = -sx * sz + cx * sy * cz;  H[1][1] = -cx * sy * sz - sx * cz;  H[1][2] = -cx * cy;
-  H[2][0] = cx * cy * cz;             H[2][1] = -cx * cy * sz;            H[2][2] = cx * sy;
-  H[3][0] = sx * cy * cz;             H[3][1] = -sx * cy * sz;            H[3][2] = sx * sy;
-  H[4][0] = -sx * cz - cx * sy * sz;  H[4][1] = sx * sz - cx * sy * cz;   H[4][2] = 0.0;
-  H[5][0] = cx * cz - sx * sy * sz;   H[5][1] = -sx * sy * cz - cx * sz;  H[5][2] = 0.0;
-  H[6][0] = -cy * cz;                 H[6][1] = cy * sz;                  H[6][2] = sy;
-  H[7][0] = -sx * sy * cz;            H[7][1] = sx * sy * sz;             H[7][2] = sx * cy;
-  H[8][0] = cx * sy * cz;             H[8][1] = -cx * sy * sz;            H[8][2] = -cx * cy;
-  H[9][0] = sy * sz;                  H[9][1] = sy * cz;                  H[9][2] = 0.0;
-  H[10][0] = -sx * cy * sz;           H[10][1] = -sx * cy * cz;           H[10][2] = 0.0;
-  H[11][0] = cx * cy * sz;            H[11][1] = cx * cy * cz;            H[11][2] = 0.0;
-  H[12][0] = -cy * cz;                H[12][1] = cy * sz;                 H[12][2] = 0.0;
-  H[13][0] = -cx * sz - sx * sy * cz; H[13][1] = -cx * cz + sx * sy * sz; H[13][2] = 0.0;
-  H[14][0] = -sx * sz + cx * sy * cz; H[14][1] = -cx * sy * sz - sx * cz; H[14][2] = 0.0;
-}
-
-// One thread: the three sin/cos pairs, then the tables.
-__device__ inline void make_pose_tables(const double* pose, PoseTables* T) {
-  double trig[6];
-  sincos(pose[3], &trig[0], &trig[1]);
-  sincos(pose[4], &trig[2], &trig[3]);
-  sincos(pose[5], &trig[4], &trig[5]);
-  make_pose_tables_trig(pose, trig, T);
-}
+// The pose tables after t (R, j, h in PoseTables order) as signed products of
+// the angles' sin/cos: entry = k0 f(a0) f(b0) f(c0) + k1 f(a1) f(b1) f(c1), with
+// f(0..5) = sin r, cos r, sin p, cos p, sin y, cos y (exact: the rotation,
+// derivatives/cpu.rs:339-372), f(6..11) the same with the snap of
+// angular.rs:39-55 (|angle| < 1e-5: sin 0, cos 1; the angular Jacobian and
+// Hessian, angular.rs:58-181) and f(12) = 1.
+struct PoseTerm {
+  int8_t k;
+  uint8_t a, b, c;
+};
+struct PoseEntry {
+  PoseTerm t[2];
+};
+constexpr int kPoseEntries = 9 + 24 + 45;
+static __constant__ const PoseEntry kPoseTable[kPoseEntries] = {
+    {{{1, 3, 5, 12}, {0, 12, 12, 12}}},  // R[0] = cp * cy
+    {{{-1, 3, 4, 12}, {0, 12, 12, 12}}},  // R[1] = -cp * sy
+    {{{1, 2, 12, 12}, {0, 12, 12, 12}}},  // R[2] = sp
+    {{{1, 0, 2, 5}, {1, 1, 4, 12}}},  // R[3] = sr * sp * cy + cr * sy
+    {{{1, 1, 5, 12}, {-1, 0, 2, 4}}},  // R[4] = cr * cy - sr * sp * sy
+    {{{-1, 0, 3, 12}, {0, 12, 12, 12}}},  // R[5] = -sr * cp
+    {{{1, 0, 4, 12}, {-1, 1, 2, 5}}},  // R[6] = sr * sy - cr * sp * cy
+    {{{1, 1, 2, 4}, {1, 0, 5, 12}}},  // R[7] = cr * sp * sy + sr * cy
+    {{{1, 1, 3, 12}, {0, 12, 12, 12}}},  // R[8] = cr * cp
+    {{{-1, 6, 10, 12}, {1, 7, 8, 11}}},  // j[0][0] = -sx * sz + cx * sy * cz
+    {{{-1, 6, 11, 12}, {-1, 7, 8, 10}}},  // j[0][1] = -sx * cz - cx * sy * sz
+    {{{-1, 7, 9, 12}, {0, 12, 12, 12}}},  // j[0][2] = -cx * cy
+    {{{1, 7, 10, 12}, {1, 6, 8, 11}}},  // j[1][0] = cx * sz + sx * sy * cz
+    {{{1, 7, 11, 12}, {-1, 6, 8, 10}}},  // j[1][1] = cx * cz - sx * sy * sz
+    {{{-1, 6, 9, 12}, {0, 12, 12, 12}}},  // j[1][2] = -sx * cy
+    {{{-1, 8, 11, 12}, {0, 12, 12, 12}}},  // j[2][0] = -sy * cz
+    {{{1, 8, 10, 12}, {0, 12, 12, 12}}},  // j[2][1] = sy * sz
+    {{{1, 9, 12, 12}, {0, 12, 12, 12}}},  // j[2][2] = cy
+    {{{1, 6, 9, 11}, {0, 12, 12, 12}}},  // j[3][0] = sx * cy * cz
+    {{{-1, 6, 9, 10}, {0, 12, 12, 12}}},  // j[3][1] = -sx * cy * sz
+    {{{1, 6, 8, 12}, {0, 12, 12, 12}}},  // j[3][2] = sx * sy
+    {{{-1, 7, 9, 11}, {0, 12, 12, 12}}},  // j[4][0] = -cx * cy * cz
+    {{{1, 7, 9, 10}, {0, 12, 12, 12}}},  // j[4][1] = cx * cy * sz
+    {{{-1, 7, 8, 12}, {0, 12, 12, 12}}},  // j[4][2] = -cx * sy
+    {{{-1, 9, 10, 12}, {0, 12, 12, 12}}},  // j[5][0] = -cy * sz
+    {{{-1, 9, 11, 12}, {0, 12, 12, 12}}},  // j[5][1] = -cy * cz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // j[5][2] = 0.0
+    {{{1, 7, 11, 12}, {-1, 6, 8, 10}}},  // j[6][0] = cx * cz - sx * sy * sz
+    {{{-1, 7, 10, 12}, {-1, 6, 8, 11}}},  // j[6][1] = -cx * sz - sx * sy * cz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // j[6][2] = 0.0
+    {{{1, 6, 11, 12}, {1, 7, 8, 10}}},  // j[7][0] = sx * cz + cx * sy * sz
+    {{{1, 7, 8, 11}, {-1, 6, 10, 12}}},  // j[7][1] = cx * sy * cz - sx * sz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // j[7][2] = 0.0
+    {{{-1, 7, 10, 12}, {-1, 6, 8, 11}}},  // h[0][0] = -cx * sz - sx * sy * cz
+    {{{-1, 7, 11, 12}, {1, 6, 8, 10}}},  // h[0][1] = -cx * cz + sx * sy * sz
+    {{{1, 6, 9, 12}, {0, 12, 12, 12}}},  // h[0][2] = sx * cy
+    {{{-1, 6, 10, 12}, {1, 7, 8, 11}}},  // h[1][0] = -sx * sz + cx * sy * cz
+    {{{-1, 7, 8, 10}, {-1, 6, 11, 12}}},  // h[1][1] = -cx * sy * sz - sx * cz
+    {{{-1, 7, 9, 12}, {0, 12, 12, 12}}},  // h[1][2] = -cx * cy
+    {{{1, 7, 9, 11}, {0, 12, 12, 12}}},  // h[2][0] = cx * cy * cz
+    {{{-1, 7, 9, 10}, {0, 12, 12, 12}}},  // h[2][1] = -cx * cy * sz
+    {{{1, 7, 8, 12}, {0, 12, 12, 12}}},  // h[2][2] = cx * sy
+    {{{1, 6, 9, 11}, {0, 12, 12, 12}}},  // h[3][0] = sx * cy * cz
+    {{{-1, 6, 9, 10}, {0, 12, 12, 12}}},  // h[3][1] = -sx * cy * sz
+    {{{1, 6, 8, 12}, {0, 12, 12, 12}}},  // h[3][2] = sx * sy
+    {{{-1, 6, 11, 12}, {-1, 7, 8, 10}}},  // h[4][0] = -sx * cz - cx * sy * sz
+    {{{1, 6, 10, 12}, {-1, 7, 8, 11}}},  // h[4][1] = sx * sz - cx * sy * cz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[4][2] = 0.0
+    {{{1, 7, 11, 12}, {-1, 6, 8, 10}}},  // h[5][0] = cx * cz - sx * sy * sz
+    {{{-1, 6, 8, 11}, {-1, 7, 10, 12}}},  // h[5][1] = -sx * sy * cz - cx * sz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[5][2] = 0.0
+    {{{-1, 9, 11, 12}, {0, 12, 12, 12}}},  // h[6][0] = -cy * cz
+    {{{1, 9, 10, 12}, {0, 12, 12, 12}}},  // h[6][1] = cy * sz
+    {{{1, 8, 12, 12}, {0, 12, 12, 12}}},  // h[6][2] = sy
+    {{{-1, 6, 8, 11}, {0, 12, 12, 12}}},  // h[7][0] = -sx * sy * cz
+    {{{1, 6, 8, 10}, {0, 12, 12, 12}}},  // h[7][1] = sx * sy * sz
+    {{{1, 6, 9, 12}, {0, 12, 12, 12}}},  // h[7][2] = sx * cy
+    {{{1, 7, 8, 11}, {0, 12, 12, 12}}},  // h[8][0] = cx * sy * cz
+    {{{-1, 7, 8, 10}, {0, 12, 12, 12}}},  // h[8][1] = -cx * sy * sz
+    {{{-1, 7, 9, 12}, {0, 12, 12, 12}}},  // h[8][2] = -cx * cy
+    {{{1, 8, 10, 12}, {0, 12, 12, 12}}},  // h[9][0] = sy * sz
+    {{{1, 8, 11, 12}, {0, 12, 12, 12}}},  // h[9][1] = sy * cz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[9][2] = 0.0
+    {{{-1, 6, 9, 10}, {0, 12, 12, 12}}},  // h[10][0] = -sx * cy * sz
+    {{{-1, 6, 9, 11}, {0, 12, 12, 12}}},  // h[10][1] = -sx * cy * cz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[10][2] = 0.0
+    {{{1, 7, 9, 10}, {0, 12, 12, 12}}},  // h[11][0] = cx * cy * sz
+    {{{1, 7, 9, 11}, {0, 12, 12, 12}}},  // h[11][1] = cx * cy * cz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[11][2] = 0.0
+    {{{-1, 9, 11, 12}, {0, 12, 12, 12}}},  // h[12][0] = -cy * cz
+    {{{1, 9, 10, 12}, {0, 12, 12, 12}}},  // h[12][1] = cy * sz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[12][2] = 0.0
+    {{{-1, 7, 10, 12}, {-1, 6, 8, 11}}},  // h[13][0] = -cx * sz - sx * sy * cz
+    {{{-1, 7, 11, 12}, {1, 6, 8, 10}}},  // h[13][1] = -cx * cz + sx * sy * sz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[13][2] = 0.0
+    {{{-1, 6, 10, 12}, {1, 7, 8, 11}}},  // h[14][0] = -sx * sz + cx * sy * cz
+    {{{-1, 7, 8, 10}, {-1, 6, 11, 12}}},  // h[14][1] = -cx * sy * sz - sx * cz
+    {{{0, 12, 12, 12}, {0, 12, 12, 12}}},  // h[14][2] = 0.0
+};
 
 // nalgebra UnitQuaternion::from_euler_angles (w, i, j, k), used by the
 // Isometry3 that scoring transforms points with (scoring/nvtl.rs:231).
@@ -626,21 +668,38 @@ constexpr int kJacobiPairs[5][3][2] = {{{0, 1}, {2, 3}, {4, 5}}, {{0, 2}, {1, 4}
 // Orders LDS accesses of the lanes of one wave (no cross-wave barrier).
 __device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
 
-// make_pose_tables by all lanes of one wave: lanes 0-2 evaluate the three
-// sin/cos pairs at once (each is a long dependent chain), lane 0 forms the
-// tables.  pose (6) and trig (6) are in LDS; pose must be written before the
-// call by a lane of the same wave.
-__device__ inline void make_pose_tables_wave(const double* pose, double* trig, PoseTables* T) {
+// The pose tables by all lanes of one wave: lanes 0-2 evaluate the three
+// sin/cos pairs at once (each is a long dependent chain), lanes 6-12 form the
+// snapped values, then every lane evaluates table entries.  pose (6) and vals
+// (13) are in LDS; pose must be written before the call by a lane of the same
+// wave.
+__device__ inline void make_pose_tables_wave(const double* pose, double* vals, PoseTables* T) {
   const int lane = threadIdx.x & (kWave - 1);
   wave_lds_fence();
   if (lane < 3) {
     double sn, cs;
     sincos(pose[3 + lane], &sn, &cs);
-    trig[2 * lane] = sn;
-    trig[2 * lane + 1] = cs;
+    vals[2 * lane] = sn;
+    vals[2 * lane + 1] = cs;
   }
   wave_lds_fence();
-  if (lane == 0) make_pose_tables_trig(pose, trig, T);
+  if (lane >= 6 && lane < 13) {
+    double v = 1.0;
+    if (lane < 12) {
+      const int q = lane - 6;  // sin (even) or cos (odd) of angle q / 2
+      v = fabs(pose[3 + q / 2]) < 1e-5 ? (double)(q & 1) : vals[q];
+    }
+    vals[lane] = v;
+  }
+  wave_lds_fence();
+  double* out = reinterpret_cast<double*>(T);
+  for (int e = lane; e < kPoseEntries; e += kWave) {
+    const PoseEntry E = kPoseTable[e];
+    const double p0 = vals[E.t[0].a] * vals[E.t[0].b] * vals[E.t[0].c];
+    const double p1 = vals[E.t[1].a] * vals[E.t[1].b] * vals[E.t[1].c];
+    out[3 + e] = (double)E.t[0].k * p0 + (double)E.t[1].k * p1;
+  }
+  if (lane < 3) out[lane] = pose[lane];
 }
 
 // Value of x in lane l (l uniform), as a scalar.

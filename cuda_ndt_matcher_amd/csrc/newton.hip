@@ -649,7 +649,7 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
   double* const s_red = s_big.red;
   const Stage stage{s_big.rec, s_stage_slot};
   __shared__ double s_q[4];
-  __shared__ double s_tp[6], s_trig[6];  // trial pose; sin/cos of the tables' angles
+  __shared__ double s_tp[6], s_trig[13];  // trial pose; sin/cos values of the pose tables
 
   const int W = sy.W;
   const int rank = sy.rank;
@@ -1286,7 +1286,7 @@ hipError_t launch_derivatives(const GridView& g, const NewtonConfig& cfg, const 
                               const double pose[6], int hess, double* d_partials, int blocks, double* d_out,
                               hipStream_t st) {
   PoseTables T;
-  // Host copy of make_pose_tables: computed on the host with the same formulas.
+  // The pose tables on the host: the formulas behind kPoseTable, in host code.
   {
     const double sr = sin(pose[3]), cr = cos(pose[3]), sp = sin(pose[4]), cp = cos(pose[4]), sy = sin(pose[5]),
                  cy = cos(pose[5]);
