@@ -840,6 +840,24 @@ __device__ inline void newton_step_wg(const double* H, const double* g, double* 
       // error of order 1e-15 times the condition of the gaps, below what the
       // step needs; one more sweep would only polish rounding.
       if (off == 0.0 || off <= 1e-30 * dg) break;
+      // Nearly diagonal (off-diagonal norm <= 1e-8 of the diagonal's): the
+      // last sweep can be replaced by the first-order eigenvector update
+      // V <- V (I + E), E_kl = A_kl / (A_ll - A_kk), whose eigenvector and
+      // orthogonality errors are O(E^2) and eigenvalue errors O(A_kl E_kl);
+      // taken only when every |E_kl| <= 1e-8, i.e. errors at rounding level.
+      if (off <= 1e-16 * dg) {
+        const double dii = __shfl(aij, 7 * i, kWave), djj = __shfl(aij, 7 * j, kWave);
+        const double gap = djj - dii;
+        const double e = (live && i != j && gap != 0.0) ? aij / gap : 0.0;
+        const bool small = !live || i == j || (gap != 0.0 && fabs(e) <= 1e-8);
+        if (__ballot(small) == ~0ull) {
+          double vn = vij;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) vn += __shfl(vij, i * 6 + k, kWave) * __shfl(e, k * 6 + j, kWave);
+          vij = live ? vn : 0.0;
+          break;
+        }
+      }
       if (__hip_atomic_load(&W.fast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
         fast_exit = true;
         break;
