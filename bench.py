@@ -2,25 +2,31 @@
 
 A step is one full alignment (NdtOptimizer::align semantics: Newton loop,
 6x6 solve, final NVTL) of the synthetic 1 km x 1 km urban workload, with the
-scan already resident in HBM.  The map is built once on the GPU before timing
-(its build time is reported separately).  Prints one JSON line.
+scan already resident in HBM.  Step i starts from trial start i mod K: K
+seeded draws (--trials, default 16; seeds listed in the JSON) of the
+configuration's guess distribution, N(0, 1 m) on x, y, z and N(0, 3 deg) on
+roll, pitch and yaw.  The map is built once on the GPU before timing (its
+build time is reported separately).  Prints one JSON line with, beside the
+throughput: per-alignment latency percentiles over at least 64 alignments,
+the status histogram over the trial starts, the Newton kernel's roofline,
+the CPU baseline (the oracle compiled for the host, one core) and per-seed
+parity rows against the oracle.
 
 Options: --gpus N (only 1 is supported: every run here uses one GPU),
---steps K, --warmup W, --no-cpu (skip the CPU baseline), --particles P
-(also time the batched initial-pose search with P particles),
---learned (train the bf16 proposal MLP on 16k scans of the 200k-point map
-and compare proposal-then-refine with the plain 200-particle search),
---serve (continuous batching of a Poisson stream of 1k-40k-point requests
-after a closed-loop autotune of the slot shape: req/s, p50/p99 latency),
---calibrate-copy MIB (run one device copy of MIB MiB before the timed
-region: a kernel of known traffic for scaling rocprofv3 FETCH_SIZE and
-WRITE_SIZE counters), --precision-sweep (the FP32 + two-sum and FP64 forms
-on 5k/15k/30k/60k-point scans of the 200k-point map: aligns/s, pose error
-against the FP64 oracle and iteration agreement).
+--steps K, --warmup W, --trials K, --no-cpu (skip the CPU baseline and the
+parity rows), --particles P (also time the batched initial-pose search with
+P particles), --learned (train the bf16 proposal MLP and compare it with the
+plain 200-particle search), --serve (continuous batching of a Poisson stream
+of 1k-40k-point requests after a closed-loop autotune of the slot shape:
+req/s, p50/p99 latency), --calibrate-copy MIB (run one device copy of MIB MiB
+before the timed region: a kernel of known traffic for scaling rocprofv3
+FETCH_SIZE and WRITE_SIZE counters), --precision-sweep (the FP32 + two-sum
+and FP64 forms on 5k/15k/30k/60k-point scans of the 200k-point map: aligns/s,
+pose error against the FP64 oracle and iteration agreement).
 
 roofline.traffic is the HBM bytes per Newton launch measured with rocprofv3
-counters; it is read from profiles/newton_kernel_traffic.md (null when that
-file has no figure).
+counters on this command; it is read from profiles/newton_kernel_traffic.md
+(null when that file has no figure) and labelled as such in the line.
 """
 import argparse
 import json
@@ -505,55 +511,129 @@ def ingest_bench(seed, with_cpu=True, reps=20):
     return out
 
 
-def cpu_baseline(wl, budget_s=12.0, grid=None):
+def host_info():
+    """CPU model and logical CPU count of the machine running the baseline."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def trial_starts(wl, seed, k):
+    """The benchmark's starts: k seeded draws of the full-size configuration's
+    guess distribution (N(0, 1 m) on x, y, z and N(0, 3 deg) on roll, pitch,
+    yaw around the true pose); trial seeds 100 + 64 * seed + j."""
+    from cuda_ndt_matcher_amd import synth
+
+    seeds = [100 + 64 * seed + j for j in range(k)]
+    return seeds, [synth.perturb(wl.true_pose, s, 1.0, 3.0, True) for s in seeds]
+
+
+def converging_starts(wl, k):
+    """Starts the reference's solver converges from on this map: the true pose
+    plus N(0, 5 cm) on x, y, z (an offset of a few tenths of a degree in roll
+    or pitch already keeps it from converging within 30 iterations)."""
+    out = []
+    for i in range(k):
+        r = np.random.default_rng(300 + i)
+        out.append(np.asarray(wl.true_pose, np.float64) + np.concatenate([r.normal(0, 0.05, 3), np.zeros(3)]))
+    return out
+
+
+def cpu_baseline(wl, starts, budget_s=12.0):
+    """The oracle (C++ FP64 restatement of the reference CPU path, which runs
+    alignments on one core) compiled for this host, aligning the same scan
+    from the same starts in turn until the budget is spent."""
     from oracle import oracle as O
 
-    g = grid if grid is not None else O.Grid(wl.map_points, 2.0)
+    lib = O.native_lib()
+    flags = O.NATIVE_FLAGS if lib is not None else "-O2 -ffp-contract=off (native build unavailable)"
+    g = O.Grid(wl.map_points, 2.0, library=lib)
     p = O.params()
     t0 = time.perf_counter()
     k = 0
     while True:
-        g.align(wl.scan, wl.guess, p)
+        g.align(wl.scan, starts[k % len(starts)], p)
         k += 1
         el = time.perf_counter() - t0
-        if (el > budget_s and k >= 3) or k >= 200:
+        if (el > budget_s and k >= 3) or k >= 400:
             break
+    model, ncpu = host_info()
     return {"value": k / el, "unit": "aligns/s", "cores": 1, "kind": "port",
-            "sample": f"{k} single-core aligns of the same {wl.scan.shape[0]}-pt scan against the same 5M-pt map "
-                      f"(C++ FP64 restatement of the reference CPU path, -O2), {el:.1f} s"}
+            "sample": f"{k} single-core aligns of the same {wl.scan.shape[0]}-pt scan against the same 5M-pt map from "
+                      f"the benchmark's starts in turn (C++ FP64 restatement of the reference CPU path, g++ {flags}), "
+                      f"{el:.1f} s",
+            "cpu_model": model, "nproc": ncpu}
 
 
-def parity_line(m, wl, grid):
-    """The oracle beside the GPU on the full-size workload.  A well-conditioned
-    start (0.4 m, 1.5 deg from the true pose): pose, rotation, iteration and
-    NVTL differences.  The bench's own start: how far the oracle's answer moves
-    when every iteration's sums are perturbed at 1e-12 (three seeds), against
-    the GPU's distance from it; that start wanders at max iterations, so the
-    two are of the same size."""
+def _oracle_pool(fn, items, threads=16):
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(fn, items))
+
+
+def parity_rows(res_by_seed, seeds, starts, wl, grid, conv_results, conv_starts):
+    """The oracle beside the GPU on the full-size workload, per trial seed:
+    status and iterations of both, the pose difference, and, where the two
+    differ beyond 1 mm / 0.01 deg, how far the oracle's own answer moves when
+    every iteration's sums are perturbed at 1e-12 (five seeds): such starts
+    are chaotic under the reference's clamped Newton step.  Plus the
+    converging starts, held to the whole bar."""
     from oracle import oracle as O
 
+    names = ["Converged", "MaxIterations", "NoCorrespondences", "SingularHessian"]
     p = O.params()
-    rng = np.random.default_rng(5)
-    start = wl.true_pose + np.array([rng.normal(0, 0.4), rng.normal(0, 0.4), 0.0, 0.0, 0.0, np.deg2rad(1.5)])
-    o = grid.align(wl.scan, start, p)
-    r = m.align(wl.scan, start)
-    rot = np.rad2deg(np.abs(np.angle(np.exp(1j * (np.asarray(r.pose_vector[3:]) - np.asarray(o.pose[3:]))))).max())
-    well = {"pose_err_mm": 1e3 * float(np.abs(np.asarray(r.pose_vector[:3]) - np.asarray(o.pose[:3])).max()),
-            "rot_err_deg": float(rot), "iterations": [int(r.iterations), int(o.iterations)],
-            "nvtl_abs_err": abs(r.nvtl - o.nvtl), "tp_abs_err": abs(r.transform_probability - o.transform_probability),
-            "status": [r.status, ["Converged", "MaxIterations", "NoCorrespondences", "SingularHessian"][o.status]]}
-    ob = grid.align(wl.scan, wl.guess, p)
-    spread = 0.0
-    for seed in (1, 2, 3):
-        q = grid.align(wl.scan, wl.guess, O.params(noise=1e-12, noise_seed=seed))
-        spread = max(spread, float(np.abs(np.asarray(q.pose[:3]) - np.asarray(ob.pose[:3])).max()))
-    rb = m.align(wl.scan, wl.guess)
-    bench = {"oracle_self_spread_m": spread,
-             "gpu_vs_oracle_m": float(np.abs(np.asarray(rb.pose_vector[:3]) - np.asarray(ob.pose[:3])).max())}
-    print(f"[bench] parity: well-conditioned start {well['pose_err_mm']:.4f} mm {well['rot_err_deg']:.5f} deg, "
-          f"iterations {well['iterations']}; bench start: oracle moves {spread:.3f} m under 1e-12 perturbations, "
-          f"GPU {bench['gpu_vs_oracle_m']:.3f} m from it", file=sys.stderr, flush=True)
-    return {"well_conditioned_start": well, "bench_start": bench}
+
+    def one(j):
+        r, start = res_by_seed[j], starts[j]
+        o = grid.align(wl.scan, start, p)
+        dt = float(np.abs(np.asarray(r.pose_vector[:3]) - o.pose[:3]).max())
+        dr = float(np.rad2deg(np.abs(np.angle(np.exp(1j * (np.asarray(r.pose_vector[3:]) - o.pose[3:])))).max()))
+        row = {"seed": seeds[j], "status": [r.status, names[o.status]], "iterations": [int(r.iterations), int(o.iterations)],
+               "pose_err_m": dt, "rot_err_deg": dr, "nvtl_abs_err": abs(r.nvtl - o.nvtl)}
+        same = r.status == names[o.status] and r.iterations == o.iterations and dt <= 1e-3 and dr <= 0.01
+        if same:
+            row["parity"] = "match"
+            return row
+        spread, statuses = 0.0, {o.status}
+        for k in range(1, 6):
+            q = grid.align(wl.scan, start, O.params(noise=1e-12, noise_seed=k))
+            statuses.add(q.status)
+            spread = max(spread, float(np.abs(q.pose[:3] - o.pose[:3]).max()))
+        row["oracle_probe_spread_m"] = spread
+        ok = r.status in {names[x] for x in statuses} and dt <= 2 * spread + 1e-3
+        row["parity"] = "within probe spread" if ok and spread > 1e-5 else "FAIL"
+        return row
+
+    rows = _oracle_pool(one, range(len(seeds)))
+
+    def conv(j):
+        r, start = conv_results[j], conv_starts[j]
+        o = grid.align(wl.scan, start, p)
+        H = np.asarray(r.hessian)
+        c6 = -np.linalg.inv(H)
+        o6 = -np.linalg.inv(o.hessian)
+        return {"status": [r.status, names[o.status]], "iterations": [int(r.iterations), int(o.iterations)],
+                "pose_err_mm": 1e3 * float(np.abs(np.asarray(r.pose_vector[:3]) - o.pose[:3]).max()),
+                "nvtl_abs_err": abs(r.nvtl - o.nvtl), "tp_abs_err": abs(r.transform_probability - o.transform_probability),
+                "hessian_rel_err": float(np.abs(H - o.hessian).max() / np.abs(o.hessian).max()),
+                "laplace6_rel_err": float(np.abs(c6 - o6).max() / np.abs(o6).max())}
+
+    crow = _oracle_pool(conv, range(len(conv_starts)))
+    hist = {}
+    for row in rows:
+        hist[row["parity"]] = hist.get(row["parity"], 0) + 1
+    print(f"[bench] parity over {len(rows)} trial seeds: {hist}; converging starts: max pose err "
+          f"{max(c['pose_err_mm'] for c in crow):.2e} mm, max Hessian rel err "
+          f"{max(c['hessian_rel_err'] for c in crow):.2e}", file=sys.stderr, flush=True)
+    return {"trial_seeds": rows, "summary": hist, "converging_starts": crow}
 
 
 def cpu_particle_baseline(map_points, cases, budget_s=12.0, threads=None):
@@ -591,8 +671,9 @@ def cpu_particle_baseline(map_points, cases, budget_s=12.0, threads=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--trials", type=int, default=16, help="trial seeds (starts) cycled through by the timed steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
@@ -646,8 +727,10 @@ def main():
     torch.cuda.synchronize()
     ptr = scan_d.data_ptr()
 
-    for _ in range(args.warmup):
-        r = m.align_device(ptr, n, wl.guess)
+    seeds, starts = trial_starts(wl, args.seed, args.trials)
+    K = len(starts)
+    for i in range(args.warmup):
+        m.align_device(ptr, n, starts[i % K])
     torch.cuda.synchronize()
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device=dev)
@@ -656,20 +739,42 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] calibration copy {args.calibrate_copy} MiB", file=sys.stderr, flush=True)
         del src, dst
-    kernel_ms = []
+    # Timed region: step i aligns the scan from trial start i mod K.
+    kernel_ms, flop_total = [], 0.0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = m.align_device(ptr, n, wl.guess)
+    for i in range(args.steps):
+        r = m.align_device(ptr, n, starts[i % K])
         kernel_ms.append(m.timing()["align_ms"])
+        flop_total += newton_flops(n, r.pair_evaluations, r.iterations, r.num_correspondences)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms_per_step = 1e3 * el / args.steps
     value = args.steps / el
     k_ms = float(np.mean(kernel_ms))
-    flops = newton_flops(n, r.pair_evaluations, r.iterations, r.num_correspondences)
-    achieved = flops / (k_ms * 1e-3) / 1e12
-    print(f"[bench] {value:.1f} aligns/s, kernel {k_ms:.3f} ms, iterations {r.iterations}, "
-          f"corr {r.num_correspondences}, status {r.status}", file=sys.stderr, flush=True)
+    achieved = flop_total / (float(np.sum(kernel_ms)) * 1e-3) / 1e12
+    # Per-alignment latency over every trial start, repeated (at least 64
+    # alignments), outside the timed region.
+    reps = max(4, -(-64 // K))
+    wall, kern, by_seed = [], [], [None] * K
+    for rep in range(reps):
+        for j in range(K):
+            t1 = time.perf_counter()
+            rr = m.align_device(ptr, n, starts[j])
+            wall.append(1e3 * (time.perf_counter() - t1))
+            kern.append(m.timing()["align_ms"])
+            by_seed[j] = rr
+    statuses = {}
+    for rr in by_seed:
+        statuses[rr.status] = statuses.get(rr.status, 0) + 1
+    latency = {"alignments": len(wall), "wall_ms_p50": float(np.percentile(wall, 50)),
+               "wall_ms_p99": float(np.percentile(wall, 99)), "kernel_ms_p50": float(np.percentile(kern, 50)),
+               "kernel_ms_p99": float(np.percentile(kern, 99)),
+               "iterations_by_seed": [int(x.iterations) for x in by_seed]}
+    print(f"[bench] {value:.1f} aligns/s over {K} trial starts, kernel mean {k_ms:.3f} ms "
+          f"(p50 {latency['kernel_ms_p50']:.3f}, p99 {latency['kernel_ms_p99']:.3f}), statuses {statuses}",
+          file=sys.stderr, flush=True)
+    conv_starts = converging_starts(wl, 6)
+    conv_results = [m.align_device(ptr, n, q) for q in conv_starts]
 
     extra = {}
     if args.phases:
@@ -724,13 +829,18 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "full: 1x1 km procedural urban map, 5M FP32 points (2.0 m voxels), 128-beam lidar "
-                               "scan downsampled at 0.3 m, guess N(0,1 m) xyz / N(0,3 deg) rpy, KDTREE-radius, "
-                               "30 iterations max",
+                               "scan downsampled at 0.3 m, starts N(0,1 m) xyz / N(0,3 deg) rpy around the true "
+                               "pose (one per trial seed, step i uses seed i mod K), KDTREE-radius, 30 iterations max",
                    "map_points": int(wl.map_points.shape[0]), "scan_points": int(n), "voxels": int(voxels),
-                   "iterations": int(r.iterations), "seed": args.seed},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "trial_seeds": seeds, "statuses": statuses, "seed": args.seed},
+        "latency": latency,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
-                     "kernel": "newton_kernel", "kernel_ms": k_ms, "flop_per_launch": flops},
+                     "traffic_source": "profile constant: rocprofv3 FETCH_SIZE + WRITE_SIZE per newton_kernel "
+                                       "dispatch of this command, scaled by a copy kernel of known size "
+                                       "(profiles/newton_kernel_traffic.md)",
+                     "kernel": "newton_kernel", "kernel_ms": k_ms,
+                     "flop_per_launch": flop_total / args.steps},
         "map_build_ms": build_ms,
         "map_build_first_ms": build_cold_ms,
         "map_build_gbs": build_bytes(wl.map_points.shape[0], voxels, 22) / (build_ms * 1e-3) / 1e9,
@@ -752,9 +862,11 @@ def main():
     if not args.no_cpu:
         from oracle import oracle as O
 
+        line["cpu_baseline"] = cpu_baseline(wl, starts)
         grid = O.Grid(wl.map_points, 2.0)
-        line["cpu_baseline"] = cpu_baseline(wl, grid=grid)
-        line["parity"] = parity_line(m, wl, grid)
+        line["parity"] = parity_rows(by_seed, seeds, starts, wl, grid, conv_results, conv_starts)
+        del grid
+    m.close()
     print(json.dumps(line), flush=True)
 
 

@@ -135,7 +135,7 @@ EXPORTS = [
     "ndt_hip_clear_regularization_pose", "ndt_hip_laplace_covariance_xy", "ndt_hip_laplace_covariance_6x6",
     "ndt_hip_get_timing", "ndt_hip_get_phase_profile", "ndt_hip_get_tuning",
     "ndt_hip_server_start", "ndt_hip_server_submit", "ndt_hip_server_poll", "ndt_hip_server_stop",
-    "ndt_hip_mlp_create", "ndt_hip_mlp_destroy", "ndt_hip_mlp_last_error", "ndt_hip_mlp_num_params",
+    "ndt_hip_server_set_idle_timeout", "ndt_hip_mlp_create", "ndt_hip_mlp_destroy", "ndt_hip_mlp_last_error", "ndt_hip_mlp_num_params",
     "ndt_hip_mlp_get_params", "ndt_hip_mlp_set_params", "ndt_hip_mlp_forward_device", "ndt_hip_mlp_forward_backward",
     "ndt_hip_mlp_train_step_device", "ndt_hip_bev_device", "ndt_hip_filter_params_default", "ndt_hip_filter_points",
     "ndt_hip_filter_points_device", "ndt_hip_voxel_downsample",
@@ -212,6 +212,7 @@ def load():
     lib.ndt_hip_server_submit.argtypes = [vp, f32p, sz, f64p, C.POINTER(C.c_uint64)]
     lib.ndt_hip_server_poll.argtypes = [vp, rp, C.POINTER(C.c_uint64), sz, C.POINTER(sz)]
     lib.ndt_hip_server_stop.argtypes = [vp]
+    lib.ndt_hip_server_set_idle_timeout.argtypes = [vp, C.c_double]
     i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
     lib.ndt_hip_mlp_create.argtypes = [i32p, C.c_int32, C.c_uint64, C.POINTER(vp)]
     lib.ndt_hip_mlp_destroy.argtypes = [vp]

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -950,9 +951,15 @@ struct ndt_server {
   std::deque<Pending> queue;
   std::vector<uint64_t> slot_ticket;  // 0: free
   std::vector<unsigned long long> slot_seq;
+  unsigned long long heartbeat = 0;
   uint64_t next_ticket = 1;
+  int wall_clock_khz = 100000;
   bool running = false;
 };
+
+// Default idle timeout of a server launch: it ends this long after the last
+// submit or poll.
+constexpr double kServerIdleSeconds = 10.0;
 
 namespace {
 
@@ -965,9 +972,34 @@ T load_acquire(const T* p) {
   return __atomic_load_n(p, __ATOMIC_ACQUIRE);
 }
 
+// Tells the launch that the host is still there (resets its idle timeout).
+void server_heartbeat(ndt_server* s) {
+  s->heartbeat += 1;
+  store_release(&s->ctl->heartbeat, s->heartbeat);
+}
+
+// The launch has ended or is ending on its idle timeout: no request posted
+// from now on would be run.
+bool server_exited(ndt_server* s) { return load_acquire(&s->ctl->exited) != 0u; }
+
+int server_exited_err(ndt_server* s) {
+  const ndt::ServerControl* c = s->ctl;
+  const double tick_s = 1e-3 / (double)s->wall_clock_khz;
+  char buf[320];
+  std::snprintf(buf, sizeof(buf),
+                "server kernel has exited (no submit or poll within its idle timeout): slot %llu waited %.3f s "
+                "against a timeout of %.3f s at heartbeat %llu (host heartbeat %llu); restart the server",
+                (unsigned long long)load_acquire(&c->exit_slot), (double)load_acquire(&c->exit_waited) * tick_s,
+                (double)load_acquire(&c->exit_timeout) * tick_s, (unsigned long long)load_acquire(&c->exit_heartbeat),
+                (unsigned long long)s->heartbeat);
+  return set_err(s->h, NDT_ERR_HIP, buf);
+}
+
 // Posts queued requests to free slots (host memory writes only: no runtime
 // call that could need the device while the server occupies it).
 int server_admit(ndt_server* s) {
+  if (s->queue.empty()) return NDT_OK;
+  if (server_exited(s)) return server_exited_err(s);
   for (int k = 0; k < s->slots && !s->queue.empty(); ++k) {
     if (s->slot_ticket[k]) continue;
     ndt_server::Pending& p = s->queue.front();
@@ -1015,13 +1047,17 @@ int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size
     return fail(e, "server points");
   std::memset(s->mb, 0, sizeof(ndt::ServerSlot) * slots);
   std::memset(s->ctl, 0, sizeof(ndt::ServerControl));
+  s->wall_clock_khz = h->wall_clock_khz > 0 ? h->wall_clock_khz : 100000;
+  s->ctl->idle_ticks = (unsigned long long)(kServerIdleSeconds * 1e3 * (double)s->wall_clock_khz);
   if ((e = s->points.reserve((size_t)slots * max_points * 12)) != hipSuccess) return fail(e, "server points");
   if ((e = s->ws.reserve(ndt::server_workspace_bytes(slots, wg_per_slot))) != hipSuccess)
     return fail(e, "server workspace");
   if ((e = hipStreamCreateWithFlags(&s->kstream, hipStreamNonBlocking)) != hipSuccess) return fail(e, "stream");
   const ndt::NewtonConfig cfg = newton_config(h, wg_per_slot);
-  // The mailbox and the workspace counters must be in place before the launch.
-  if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e, "server setup");
+  // The map (built on the handle's stream) must be complete before the
+  // launch reads it; the mailbox is host memory written above, and the
+  // workspace counters are cleared on the server's own stream ahead of it.
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return fail(e, "server setup");
   if ((e = ndt::launch_server(h->grid.view, cfg, s->mb, s->ctl, slots, s->host_points,
                               reinterpret_cast<float*>(s->points.ptr), (uint32_t)max_points, s->ws.ptr, s->kstream)) !=
       hipSuccess)
@@ -1038,6 +1074,8 @@ int ndt_hip_server_submit(ndt_server* s, const float* xyz, size_t n, const doubl
   if (n == 0) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
   if (n > s->max_points) return set_err(h, NDT_ERR_CAPACITY, "request exceeds the server's points per slot");
   if (!s->running) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "server is not running");
+  server_heartbeat(s);
+  if (server_exited(s)) return server_exited_err(s);
   ndt_server::Pending p;
   p.ticket = s->next_ticket++;
   p.xyz.assign(xyz, xyz + 3 * n);
@@ -1051,6 +1089,7 @@ int ndt_hip_server_poll(ndt_server* s, ndt_result* results, uint64_t* tickets, s
   if (!s || !count || (cap && (!results || !tickets))) return NDT_ERR_INVALID_ARGUMENT;
   ndt_handle* h = s->h;
   *count = 0;
+  server_heartbeat(s);
   for (int k = 0; k < s->slots && *count < cap; ++k) {
     if (!s->slot_ticket[k]) continue;
     if (load_acquire(&s->mb[k].done) != s->slot_seq[k]) continue;
@@ -1063,8 +1102,16 @@ int ndt_hip_server_poll(ndt_server* s, ndt_result* results, uint64_t* tickets, s
   if (rc) return rc;
   bool busy = !s->queue.empty();
   for (int k = 0; k < s->slots; ++k) busy |= s->slot_ticket[k] != 0;
-  if (busy && hipStreamQuery(s->kstream) == hipSuccess)
-    return set_err(h, NDT_ERR_HIP, "server kernel has exited (idle timeout); restart the server");
+  if (busy && (server_exited(s) || hipStreamQuery(s->kstream) == hipSuccess)) return server_exited_err(s);
+  (void)h;
+  return NDT_OK;
+}
+
+int ndt_hip_server_set_idle_timeout(ndt_server* s, double seconds) {
+  if (!s) return NDT_ERR_INVALID_ARGUMENT;
+  if (!(seconds > 0.0) || !(seconds <= 3600.0)) return set_err(s->h, NDT_ERR_INVALID_ARGUMENT, "idle timeout");
+  server_heartbeat(s);
+  store_release(&s->ctl->idle_ticks, (unsigned long long)(seconds * 1e3 * (double)s->wall_clock_khz));
   return NDT_OK;
 }
 
@@ -1079,7 +1126,8 @@ int ndt_hip_server_stop(ndt_server* s) {
       bool busy = false;
       for (int k = 0; k < s->slots; ++k)
         if (s->slot_ticket[k] && load_acquire(&s->mb[k].done) != s->slot_seq[k]) busy = true;
-      if (!busy || hipStreamQuery(s->kstream) == hipSuccess) break;
+      if (!busy || server_exited(s) || hipStreamQuery(s->kstream) == hipSuccess) break;
+      server_heartbeat(s);
     }
     store_release(&s->ctl->stop, 1u);
     hipError_t e = hipStreamSynchronize(s->kstream);

@@ -129,14 +129,27 @@ struct alignas(128) ServerSlot {
   double pose[6];
   ndt_result result;
 };
+// Launch-wide control words, in pinned host memory.  The whole launch ends
+// together: on stop, or when the host has not touched the server (no submit
+// or poll, which bump heartbeat) for idle_ticks of the device wall clock.
+// The first slot that sees the idle timeout sets exited; every other slot
+// leaves when it next looks for a job, and the host refuses new requests
+// once it reads exited.
 struct alignas(128) ServerControl {
-  uint32_t stop;
+  uint32_t stop;                 // host -> device
+  uint32_t exited;               // device -> host
+  unsigned long long heartbeat;  // host -> device: bumped by every submit and poll
+  unsigned long long idle_ticks; // host -> device: idle timeout in wall-clock ticks
+  // device -> host, written before exited by the slot that timed out: its
+  // index, the wall-clock ticks it waited, the timeout it applied and the
+  // heartbeat it last saw.
+  unsigned long long exit_slot, exit_waited, exit_timeout, exit_heartbeat;
 };
 size_t server_workspace_bytes(int slots, int wg_per_slot);
 // Launches the persistent server kernel (cooperative): slots x wg_per_slot
 // workgroups.  host_points (pinned) and slot_points (device) each hold
-// slots x max_points x 3 floats.  A slot's workgroup 0 leaves the loop on
-// stop or after about 10 s without a job; the others follow it.
+// slots x max_points x 3 floats.  A slot's workgroup 0 decides for its slot
+// (run the next job or leave); the slot's other workgroups follow it.
 hipError_t launch_server(const GridView& g, const NewtonConfig& cfg, ServerSlot* mailbox, ServerControl* ctl,
                          int slots, const float* host_points, float* slot_points, uint32_t max_points, void* ws,
                          hipStream_t st);

@@ -935,13 +935,45 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 // mailbox one after another (continuous batching: a slot that finishes takes
 // the next request while the others keep running).
 //
-// Workgroup 0 of a slot polls the host mailbox (bounded: it gives up after
-// kServerIdleSpins polls without a job) and announces its decision in the
-// slot's command word in device memory: the job number to run, or kExitCmd.
-// The other workgroups of the slot follow that word, so a slot never splits
-// between running a job and leaving.
-constexpr unsigned long long kServerIdleSpins = 1ull << 21;  // ~10 s at ~5 us per poll
+// Workgroup 0 of a slot polls the host mailbox and announces its decision in
+// the slot's command word in device memory: the job number to run, or
+// kExitCmd.  The other workgroups of the slot follow that word, so a slot
+// never splits between running a job and leaving.  Leaving is launch-wide
+// (ServerControl): on stop, when another slot has set exited, or after
+// idle_ticks of the wall clock in which the host's heartbeat did not move.
 constexpr unsigned long long kExitCmd = ~0ull;
+
+// The next command of a slot's workgroup 0: done + 1 when a job is posted,
+// kExitCmd when the launch ends.  Bounded: the heartbeat timeout ends every
+// wait once the host stops calling.
+__device__ unsigned long long server_next_command(const ServerSlot* mb, ServerControl* ctl, unsigned long long done,
+                                                  int slot) {
+  unsigned long long hb = __hip_atomic_load(&ctl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  unsigned long long t0 = wall_clock64();
+  for (;;) {
+    if (__hip_atomic_load(&mb->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) > done) return done + 1;
+    if (__hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ||
+        __hip_atomic_load(&ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+      return kExitCmd;
+    const unsigned long long h = __hip_atomic_load(&ctl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const unsigned long long now = wall_clock64();
+    if (h != hb) {
+      hb = h;
+      t0 = now;
+    } else {
+      const unsigned long long limit = __hip_atomic_load(&ctl->idle_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (now - t0 > limit) {
+        __hip_atomic_store(&ctl->exit_slot, (unsigned long long)slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&ctl->exit_waited, now - t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&ctl->exit_timeout, limit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&ctl->exit_heartbeat, hb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&ctl->exited, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        return kExitCmd;
+      }
+    }
+    __builtin_amdgcn_s_sleep(127);
+  }
+}
 
 // The host leaves each request's points in pinned memory (host_points); at
 // the start of a job every workgroup copies its own point range into the
@@ -949,7 +981,7 @@ constexpr unsigned long long kExitCmd = ~0ull;
 // is needed while the launch occupies the device.
 template <bool kF32>
 __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig cfg, ServerSlot* __restrict__ mb,
-                                                        const ServerControl* __restrict__ ctl,
+                                                        ServerControl* __restrict__ ctl,
                                                         const float* __restrict__ host_points,
                                                         float* __restrict__ slot_points, uint32_t max_points,
                                                         double* __restrict__ partials, uint32_t* __restrict__ counters,
@@ -964,27 +996,24 @@ __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig
   unsigned long long done = 0;
   for (;;) {
     if (threadIdx.x == 0) {
-      unsigned long long c = kExitCmd;
+      unsigned long long c;
       if (rank == 0) {
-        for (unsigned long long spin = 0; spin < kServerIdleSpins; ++spin) {
-          if (__hip_atomic_load(&mb[slot].seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) > done) {
-            c = done + 1;
-            break;
-          }
-          if (__hip_atomic_load(&ctl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) break;
-          __builtin_amdgcn_s_sleep(127);
-        }
-        __hip_atomic_store(my_cmd, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c = server_next_command(&mb[slot], ctl, done, slot);
+        // Acquire the host's writes (points, n, pose, released with seq)
+        // before publishing the command with release semantics, so the
+        // followers' acquire of the command orders their reads after them.
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        __hip_atomic_store(my_cmd, c, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        // Workgroup 0 always writes the command word within its bounded poll.
+        // Workgroup 0 writes the command word within its bounded wait.
         for (;;) {
-          c = __hip_atomic_load(my_cmd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          c = __hip_atomic_load(my_cmd, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
           if (c > done) break;
           __builtin_amdgcn_s_sleep(8);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
       }
       if (c != kExitCmd) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");  // the host's job fields and points
         s_job.pts = slot_points + (size_t)slot * max_points * 3;
         s_job.n = mb[slot].n;
         s_job.pad = 0;

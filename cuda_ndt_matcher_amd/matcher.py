@@ -9,6 +9,7 @@ meaning and error classes.  Every call goes to the HIP library.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Tuple
 
@@ -261,6 +262,11 @@ class NdtScanMatcher:
             self._h = None
 
     def __del__(self):
+        # At interpreter shutdown the HIP runtime (and a profiler's hooks) may
+        # already be torn down: the handle is left to the process exit then.
+        # Callers release handles with close() (or a with block) before that.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -497,6 +503,11 @@ class AlignServer:
         cnt = C.c_size_t()
         self._m._check(self._lib.ndt_hip_server_poll(self._s, self._res, self._tk, self._cap, C.byref(cnt)))
         return list(zip([int(t) for t in self._tk[:cnt.value]], _results(self._res, cnt.value)))
+
+    def set_idle_timeout(self, seconds: float) -> None:
+        """The launch ends when no submit or poll comes for this long (10 s by
+        default); submit and poll then raise NdtError (HIP)."""
+        self._m._check(self._lib.ndt_hip_server_set_idle_timeout(self._s, float(seconds)))
 
     def stop(self) -> None:
         if self._s:

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
@@ -42,11 +43,17 @@ class ProposalNet:
             raise RuntimeError(f"ndt_hip_mlp_create failed ({rc})")
         self._h = h
 
-    def __del__(self):
+    def close(self) -> None:
         h = getattr(self, "_h", None)
         if h:
             self._lib.ndt_hip_mlp_destroy(h)
             self._h = None
+
+    def __del__(self):
+        # No HIP calls during interpreter shutdown (the runtime may be gone).
+        if sys.is_finalizing():
+            return
+        self.close()
 
     def _check(self, rc: int) -> None:
         if rc != 0:

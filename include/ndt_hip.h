@@ -289,13 +289,18 @@ int ndt_hip_voxel_downsample(ndt_handle* h, const float* xyz, size_t n, float re
  * ndt_hip_align of the same scan with workgroups_per_alignment = wg_per_slot.
  * While a server runs it holds the device's co-resident workgroups, so the
  * handle's other calls (and a second start) return NDT_ERR_INVALID_ARGUMENT
- * until stop; destroying the handle stops it.  A slot left idle for about
- * 10 s ends the
- * launch; poll then reports NDT_ERR_HIP and the server must be restarted. */
+ * until stop; destroying the handle stops it.  The whole launch ends when no
+ * submit or poll has been made for the idle timeout (10 s by default): every
+ * slot leaves, submit and poll then return NDT_ERR_HIP, and the server must
+ * be stopped and started again.  A caller that waits for requests keeps the
+ * launch alive by polling. */
 typedef struct ndt_server ndt_server;
 int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size_t max_points, ndt_server** out);
 int ndt_hip_server_submit(ndt_server* s, const float* xyz, size_t n, const double pose[6], uint64_t* ticket);
 int ndt_hip_server_poll(ndt_server* s, ndt_result* results, uint64_t* tickets, size_t cap, size_t* count);
+/* Sets the idle timeout (0 < seconds <= 3600) of a running server; counts
+ * from this call. */
+int ndt_hip_server_set_idle_timeout(ndt_server* s, double seconds);
 int ndt_hip_server_stop(ndt_server* s);
 
 /* Learned initial-pose proposal.  No counterpart in the reference: it seeds

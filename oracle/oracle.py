@@ -56,14 +56,18 @@ class Result(C.Structure):
 SCORE_FN = C.CFUNCTYPE(None, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
 
 
+NATIVE_LIB_PATH = os.path.join(_HERE, "libndt_oracle_native.so")
+NATIVE_FLAGS = "-O3 -march=native -ffp-contract=off"
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def _load():
-    if not os.path.exists(_LIB_PATH):
+def _load(path=_LIB_PATH):
+    if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
         build()
-    lib = C.CDLL(_LIB_PATH)
+    lib = C.CDLL(path)
     f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
     i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -107,6 +111,7 @@ def _load():
 
 
 _lib = None
+_native = None
 
 
 def lib():
@@ -114,6 +119,21 @@ def lib():
     if _lib is None:
         _lib = _load()
     return _lib
+
+
+def native_lib():
+    """The same restatement built for the host it runs on (-O3 -march=native,
+    still without contraction, so its results equal the parity build's): the
+    CPU-baseline timing build.  Compiled on first use (about 10 s); returns
+    None when the compiler is unavailable."""
+    global _native
+    if _native is None:
+        try:
+            subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True, capture_output=True)
+            _native = _load(NATIVE_LIB_PATH)
+        except (OSError, subprocess.CalledProcessError):
+            return None
+    return _native
 
 
 def _pts(points) -> np.ndarray:
@@ -183,19 +203,22 @@ class OracleResult:
 class Grid:
     """Voxel grid built by the oracle (voxels sorted by coordinate)."""
 
-    def __init__(self, points, resolution: float = 2.0, min_points: int = 6, eigen_ratio: float = 0.01):
+    def __init__(self, points, resolution: float = 2.0, min_points: int = 6, eigen_ratio: float = 0.01,
+                 library=None):
         p = _pts(points)
         self.resolution = float(resolution)
-        self._h = lib().oracle_grid_build(p, p.shape[0], np.float32(resolution), min_points, np.float32(eigen_ratio))
+        self._lib = library if library is not None else lib()
+        self._h = self._lib.oracle_grid_build(p, p.shape[0], np.float32(resolution), min_points,
+                                              np.float32(eigen_ratio))
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().oracle_grid_free(h)
+            self._lib.oracle_grid_free(h)
             self._h = None
 
     def __len__(self) -> int:
-        return int(lib().oracle_grid_size(self._h))
+        return int(self._lib.oracle_grid_size(self._h))
 
     def export(self) -> dict:
         v = len(self)
@@ -203,7 +226,7 @@ class Grid:
                    covs=np.zeros((v, 3, 3), np.float32), invs=np.zeros((v, 3, 3), np.float32),
                    axes=np.zeros((v, 3), np.float32), counts=np.zeros(v, np.uint32),
                    means64=np.zeros((v, 3), np.float64), invs64=np.zeros((v, 3, 3), np.float64))
-        lib().oracle_grid_export(self._h, out["coords"], out["means"], out["covs"], out["invs"], out["axes"],
+        self._lib.oracle_grid_export(self._h, out["coords"], out["means"], out["covs"], out["invs"], out["axes"],
                                  out["counts"], out["means64"], out["invs64"])
         return out
 
@@ -212,7 +235,7 @@ class Grid:
         n = q.shape[0]
         cnt = np.zeros(n, np.uint32)
         idx = np.zeros((n, max_k), np.int32)
-        lib().oracle_search(self._h, mode, q, n, cnt, idx, max_k)
+        self._lib.oracle_search(self._h, mode, q, n, cnt, idx, max_k)
         return cnt, idx
 
     def derivatives(self, points, pose, p: Params, compute_hessian: bool = True):
@@ -221,33 +244,33 @@ class Grid:
         H = np.zeros(36)
         s = C.c_double()
         c = C.c_uint32()
-        lib().oracle_derivatives(self._h, pts, pts.shape[0], _pose(pose), C.byref(p), int(compute_hessian),
+        self._lib.oracle_derivatives(self._h, pts, pts.shape[0], _pose(pose), C.byref(p), int(compute_hessian),
                                  C.byref(s), g, H, C.byref(c))
         return s.value, g, H.reshape(6, 6), c.value
 
     def align(self, points, pose, p: Params) -> OracleResult:
         pts = _pts(points)
         r = Result()
-        lib().oracle_align(self._h, pts, pts.shape[0], _pose(pose), C.byref(p), C.byref(r))
+        self._lib.oracle_align(self._h, pts, pts.shape[0], _pose(pose), C.byref(p), C.byref(r))
         return OracleResult(np.array(r.pose[:]), r.status, r.iterations, r.score, r.transform_probability, r.nvtl,
                             np.array(r.hessian[:]).reshape(6, 6), r.num_correspondences, r.oscillation_count)
 
     def nvtl(self, points, pose, p: Params) -> float:
         pts = _pts(points)
-        return lib().oracle_nvtl(self._h, pts, pts.shape[0], _pose(pose), C.byref(p))
+        return self._lib.oracle_nvtl(self._h, pts, pts.shape[0], _pose(pose), C.byref(p))
 
     def transform_probability(self, points, pose, p: Params) -> float:
         pts = _pts(points)
-        return lib().oracle_transform_probability(self._h, pts, pts.shape[0], _pose(pose), C.byref(p))
+        return self._lib.oracle_transform_probability(self._h, pts, pts.shape[0], _pose(pose), C.byref(p))
 
     def nvtl_cell27(self, points, pose, p: Params) -> float:
         pts = _pts(points)
-        return lib().oracle_nvtl_cell27(self._h, pts, pts.shape[0], _pose(pose), C.byref(p))
+        return self._lib.oracle_nvtl_cell27(self._h, pts, pts.shape[0], _pose(pose), C.byref(p))
 
     def per_point_scores(self, points, pose, p: Params) -> np.ndarray:
         pts = _pts(points)
         out = np.zeros(pts.shape[0])
-        lib().oracle_per_point_scores(self._h, pts, pts.shape[0], _pose(pose), C.byref(p), out)
+        self._lib.oracle_per_point_scores(self._h, pts, pts.shape[0], _pose(pose), C.byref(p), out)
         return out
 
 

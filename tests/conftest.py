@@ -19,3 +19,16 @@ def oracle_mod():
 
     O.lib()
     return O
+
+
+@pytest.fixture(scope="session")
+def full_wl():
+    """The full-size workload: 5M-point map, ~29k-point scan."""
+    from cuda_ndt_matcher_amd import synth
+
+    return synth.make_workload("full", 0)
+
+
+@pytest.fixture(scope="session")
+def full_oracle(full_wl, oracle_mod):
+    return oracle_mod.Grid(full_wl.map_points, 2.0)

@@ -115,3 +115,40 @@ def test_handle_refuses_other_work_while_serving(small):
         srv.stop()
     r = m.align(small.scan, small.guess)
     assert np.array_equal(r.pose_vector, want.pose_vector) and got[t].iterations == want.iterations
+
+
+def test_server_idle_timeout_is_launch_wide(small):
+    """With a short idle timeout, a slot that never gets work does not leave
+    while the host keeps calling (requests keep landing on slot 0 one at a
+    time); a later burst still reaches both slots.  Once the host stops
+    calling for longer than the timeout, the whole launch ends: submit and
+    poll raise, stop succeeds, and the handle works again."""
+    from cuda_ndt_matcher_amd import NdtError, NdtScanMatcher, NdtScanMatcherConfig
+
+    _print_case("server idle timeout")
+    m = NdtScanMatcher(NdtScanMatcherConfig(workgroups_per_alignment=W))
+    m.set_target(small.map_points)
+    scan = np.ascontiguousarray(small.scan[:800])
+    want = m.align(scan, small.guess)
+    srv = m.server(slots=2, workgroups_per_slot=W, max_points=scan.shape[0])
+    try:
+        srv.set_idle_timeout(0.25)
+        t_end = time.time() + 1.5  # six timeouts' worth of serving on one slot
+        n_done = 0
+        while time.time() < t_end:
+            t = srv.submit(scan, small.guess)
+            got = _drain(srv, {}, 1, time.time() + 30)
+            assert t in got and got[t].iterations == want.iterations
+            n_done += 1
+        burst = [srv.submit(scan, small.guess) for _ in range(2)]
+        got = _drain(srv, {}, 2, time.time() + 30)
+        assert set(burst) <= set(got)
+        time.sleep(1.0)  # no calls: the launch ends
+        with pytest.raises(NdtError) as e:
+            srv.submit(scan, small.guess)
+        assert e.value.code == -5
+    finally:
+        srv.stop()
+    assert n_done >= 3
+    r = m.align(scan, small.guess)
+    assert np.array_equal(r.pose_vector, want.pose_vector)
