@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -952,6 +953,8 @@ struct ndt_server {
   std::vector<uint64_t> slot_ticket;  // 0: free
   std::vector<unsigned long long> slot_seq;
   unsigned long long heartbeat = 0;
+  std::chrono::steady_clock::time_point last_call{};
+  double max_gap_s = 0.0;  // longest time between two host calls (diagnostics)
   uint64_t next_ticket = 1;
   int wall_clock_khz = 100000;
   bool running = false;
@@ -976,6 +979,10 @@ T load_acquire(const T* p) {
 void server_heartbeat(ndt_server* s) {
   s->heartbeat += 1;
   store_release(&s->ctl->heartbeat, s->heartbeat);
+  const auto now = std::chrono::steady_clock::now();
+  if (s->heartbeat > 1)
+    s->max_gap_s = std::max(s->max_gap_s, std::chrono::duration<double>(now - s->last_call).count());
+  s->last_call = now;
 }
 
 // The launch has ended or is ending on its idle timeout: no request posted
@@ -985,13 +992,14 @@ bool server_exited(ndt_server* s) { return load_acquire(&s->ctl->exited) != 0u; 
 int server_exited_err(ndt_server* s) {
   const ndt::ServerControl* c = s->ctl;
   const double tick_s = 1e-3 / (double)s->wall_clock_khz;
-  char buf[320];
+  char buf[384];
   std::snprintf(buf, sizeof(buf),
                 "server kernel has exited (no submit or poll within its idle timeout): slot %llu waited %.3f s "
-                "against a timeout of %.3f s at heartbeat %llu (host heartbeat %llu); restart the server",
+                "against a timeout of %.3f s at heartbeat %llu (host heartbeat %llu, longest host gap between "
+                "calls %.3f s); restart the server",
                 (unsigned long long)load_acquire(&c->exit_slot), (double)load_acquire(&c->exit_waited) * tick_s,
                 (double)load_acquire(&c->exit_timeout) * tick_s, (unsigned long long)load_acquire(&c->exit_heartbeat),
-                (unsigned long long)s->heartbeat);
+                (unsigned long long)s->heartbeat, s->max_gap_s);
   return set_err(s->h, NDT_ERR_HIP, buf);
 }
 
@@ -1102,7 +1110,11 @@ int ndt_hip_server_poll(ndt_server* s, ndt_result* results, uint64_t* tickets, s
   if (rc) return rc;
   bool busy = !s->queue.empty();
   for (int k = 0; k < s->slots; ++k) busy |= s->slot_ticket[k] != 0;
-  if (busy && (server_exited(s) || hipStreamQuery(s->kstream) == hipSuccess)) return server_exited_err(s);
+  // Liveness comes from the launch's own exited word.  A poll loop that also
+  // called hipStreamQuery on the server's stream was seen to stall in it
+  // until the launch ended: the stalled loop stops the heartbeat, and the
+  // idle timeout then ends the launch.
+  if (busy && server_exited(s)) return server_exited_err(s);
   (void)h;
   return NDT_OK;
 }
@@ -1122,11 +1134,16 @@ int ndt_hip_server_stop(ndt_server* s) {
   if (s->running) {
     // Requests still on a slot finish first; queued ones are dropped.
     s->queue.clear();
+    // Bounded: a slot that stops answering (the launch gone) is waited for
+    // no longer than the idle timeout plus a second.
+    const double wait_s = (double)load_acquire(&s->ctl->idle_ticks) * 1e-3 / (double)s->wall_clock_khz + 1.0;
+    const auto t0 = std::chrono::steady_clock::now();
     for (;;) {
       bool busy = false;
       for (int k = 0; k < s->slots; ++k)
         if (s->slot_ticket[k] && load_acquire(&s->mb[k].done) != s->slot_seq[k]) busy = true;
-      if (!busy || server_exited(s) || hipStreamQuery(s->kstream) == hipSuccess) break;
+      if (!busy || server_exited(s)) break;
+      if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > wait_s) break;
       server_heartbeat(s);
     }
     store_release(&s->ctl->stop, 1u);

@@ -130,8 +130,9 @@ struct alignas(128) ServerSlot {
   ndt_result result;
 };
 // Launch-wide control words, in pinned host memory.  The whole launch ends
-// together: on stop, or when the host has not touched the server (no submit
-// or poll, which bump heartbeat) for idle_ticks of the device wall clock.
+// together: on stop, or when for idle_ticks of the device wall clock the host
+// has not touched the server (no submit or poll, which bump heartbeat) and no
+// slot has finished a job.
 // The first slot that sees the idle timeout sets exited; every other slot
 // leaves when it next looks for a job, and the host refuses new requests
 // once it reads exited.

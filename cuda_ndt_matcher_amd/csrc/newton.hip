@@ -940,15 +940,18 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
 // kExitCmd.  The other workgroups of the slot follow that word, so a slot
 // never splits between running a job and leaving.  Leaving is launch-wide
 // (ServerControl): on stop, when another slot has set exited, or after
-// idle_ticks of the wall clock in which the host's heartbeat did not move.
+// idle_ticks of the wall clock in which neither the host's heartbeat nor the
+// launch's count of finished jobs moved (a slot that serves keeps the idle
+// ones alive).
 constexpr unsigned long long kExitCmd = ~0ull;
 
 // The next command of a slot's workgroup 0: done + 1 when a job is posted,
 // kExitCmd when the launch ends.  Bounded: the heartbeat timeout ends every
 // wait once the host stops calling.
 __device__ unsigned long long server_next_command(const ServerSlot* mb, ServerControl* ctl, unsigned long long done,
-                                                  int slot) {
+                                                  int slot, const unsigned long long* activity) {
   unsigned long long hb = __hip_atomic_load(&ctl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  unsigned long long act = __hip_atomic_load(activity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   unsigned long long t0 = wall_clock64();
   for (;;) {
     if (__hip_atomic_load(&mb->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) > done) return done + 1;
@@ -956,9 +959,11 @@ __device__ unsigned long long server_next_command(const ServerSlot* mb, ServerCo
         __hip_atomic_load(&ctl->exited, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
       return kExitCmd;
     const unsigned long long h = __hip_atomic_load(&ctl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const unsigned long long a = __hip_atomic_load(activity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long now = wall_clock64();
-    if (h != hb) {
+    if (h != hb || a != act) {  // a host call, or another slot finished a job
       hb = h;
+      act = a;
       t0 = now;
     } else {
       const unsigned long long limit = __hip_atomic_load(&ctl->idle_ticks, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -991,6 +996,8 @@ __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig
   const int rank = blockIdx.x % W;
   SlotSync sy = make_slot_sync(partials, counters, slot, W, rank);
   unsigned long long* my_cmd = cmd + (size_t)slot * kCounterStride / 2;
+  // One line after the command words: jobs finished by the launch.
+  unsigned long long* activity = cmd + (size_t)gridDim.x / W * kCounterStride / 2;
   __shared__ unsigned long long s_cmd;
   __shared__ SlotJob s_job;
   unsigned long long done = 0;
@@ -998,7 +1005,7 @@ __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig
     if (threadIdx.x == 0) {
       unsigned long long c;
       if (rank == 0) {
-        c = server_next_command(&mb[slot], ctl, done, slot);
+        c = server_next_command(&mb[slot], ctl, done, slot, activity);
         // Acquire the host's writes (points, n, pose, released with seq)
         // before publishing the command with release semantics, so the
         // followers' acquire of the command orders their reads after them.
@@ -1034,8 +1041,10 @@ __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig
     __syncthreads();
     align_slot<kF32>(g, cfg, job, &mb[slot].result, sy);
     ++done;
-    if (rank == 0 && threadIdx.x == 0)
+    if (rank == 0 && threadIdx.x == 0) {
       __hip_atomic_store(&mb[slot].done, done, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_fetch_add(activity, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1287,7 +1296,8 @@ hipError_t build_newton_graph(const GridView& g, const NewtonConfig& cfg, SlotJo
 }
 
 size_t server_workspace_bytes(int slots, int wg_per_slot) {
-  return newton_workspace_bytes(slots, wg_per_slot) + (size_t)slots * kCounterStride * sizeof(uint32_t);
+  // counters, then a command word line per slot, then the activity line
+  return newton_workspace_bytes(slots, wg_per_slot) + (size_t)(slots + 1) * kCounterStride * sizeof(uint32_t);
 }
 
 hipError_t launch_server(const GridView& g, const NewtonConfig& cfg, ServerSlot* mailbox, ServerControl* ctl,
@@ -1299,7 +1309,7 @@ hipError_t launch_server(const GridView& g, const NewtonConfig& cfg, ServerSlot*
       reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
   const int nc = slots * counter_lines(W) * kCounterStride;
   unsigned long long* cmd = reinterpret_cast<unsigned long long*>(counters + nc);
-  const int nall = nc + slots * kCounterStride;  // counters and command words
+  const int nall = nc + (slots + 1) * kCounterStride;  // counters, command words and the activity line
   hipLaunchKernelGGL(zero_u32, dim3(std::min(1024, (nall + 255) / 256)), dim3(256), 0, st, counters, nall);
   GridView gg = g;
   NewtonConfig cc = cfg;

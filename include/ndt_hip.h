@@ -293,7 +293,11 @@ int ndt_hip_voxel_downsample(ndt_handle* h, const float* xyz, size_t n, float re
  * submit or poll has been made for the idle timeout (10 s by default): every
  * slot leaves, submit and poll then return NDT_ERR_HIP, and the server must
  * be stopped and started again.  A caller that waits for requests keeps the
- * launch alive by polling. */
+ * launch alive by polling, and a slot that finishes a job keeps the idle
+ * slots alive.  hipFree waits for every launch on the device to end: while
+ * a server runs, freeing device memory anywhere in the process (destroying
+ * another handle, a call that grows a handle's buffers) blocks until the
+ * server ends, so such calls belong before start or after stop. */
 typedef struct ndt_server ndt_server;
 int ndt_hip_server_start(ndt_handle* h, int32_t slots, int32_t wg_per_slot, size_t max_points, ndt_server** out);
 int ndt_hip_server_submit(ndt_server* s, const float* xyz, size_t n, const double pose[6], uint64_t* ticket);

@@ -5,8 +5,9 @@
   NVTL/TP, Hessian) plus the Laplace covariances formed from the GPU Hessian;
   starts from the configuration's own distribution (N(0, 1 m) translation,
   N(0, 3 deg) roll/pitch/yaw), which the oracle does not converge from, are
-  held to the bar where the oracle is stable and to its probe spread where it
-  is not (tests/parity.py).
+  held to the bar where the oracle is stable and, where its trajectory is
+  chaotic, to what the solver reports for the trajectory taken
+  (oracle/parity.py).
 * FP32 precision variant on 5k/15k/30k/60k-point scans against the FP64
   oracle.
 * 200 particles on the 5M-point map in one graph-captured launch.
@@ -15,6 +16,7 @@
   alone and meets the oracle.
 * The SingularHessian exit (solver.rs:196-227) on a case built to reach it.
 """
+import gc
 import math
 import sys
 import time
@@ -22,7 +24,7 @@ import time
 import numpy as np
 import pytest
 
-from tests import parity
+from oracle import parity
 
 pytestmark = pytest.mark.gpu
 
@@ -92,8 +94,9 @@ def test_full_converged_starts(full_gpu, full_wl, full_oracle, oracle_mod):
 
 def test_full_config_starts_bounded(full_gpu, full_wl, full_oracle, oracle_mod):
     """Starts drawn like the benchmark's (seeds 100..103): the GPU meets the
-    bar or, where the oracle itself is chaotic, stays within its probe
-    spread with a status the probe shows."""
+    bar or, where the oracle's own trajectory is chaotic, reports a status
+    the probe shows, the same iteration count, and the score and NVTL the
+    oracle gives at the reported pose."""
     from cuda_ndt_matcher_amd import synth
 
     p = oracle_mod.params()
@@ -110,7 +113,7 @@ def test_precision_sweep_sizes(size_index, oracle_mod):
     oracle on the sweep's scans (5k/15k/30k/60k points of a 128-beam scan of
     the 200k-point map): converging starts to 1 mm / 0.01 deg with the same
     iterations and status; the sweep's own start (not converging) within the
-    probe spread (noise 1e-8 for FP32 terms, 1e-12 for FP64)."""
+    probe spread (noise 1e-6 for FP32 terms, 1e-10 for FP64)."""
     from cuda_ndt_matcher_amd import synth
 
     sizes = (5000, 15000, 30000, 60000)
@@ -119,7 +122,7 @@ def test_precision_sweep_sizes(size_index, oracle_mod):
     og = oracle_mod.Grid(wl.map_points, 2.0)
     p = oracle_mod.params()
     starts = converging_starts(wl.true_pose, 800, 2)
-    for prec, noise, hrel in ((1, 1e-8, 1e-4), (0, 1e-12, 1e-6)):
+    for prec, noise, hrel, srel in ((1, 1e-6, 1e-4, 2e-5), (0, 1e-10, 1e-6, 1e-9)):
         m = _matcher(wl.map_points, precision=prec)
         for k, start in enumerate(starts):
             _print_case(f"sweep {sizes[size_index]} precision {prec} start {k}")
@@ -128,7 +131,8 @@ def test_precision_sweep_sizes(size_index, oracle_mod):
             why = parity.mismatch(m.align(scan, start), o, hessian_rel=hrel)
             assert why is None, f"{sizes[size_index]} points, precision {prec}, start {k}: {why}"
         _print_case(f"sweep {sizes[size_index]} precision {prec} guess")
-        parity.check_alignment(m.align(scan, wl.guess), og, scan, wl.guess, p, noise=noise, hessian_rel=hrel)
+        parity.check_alignment(m.align(scan, wl.guess), og, scan, wl.guess, p, noise=noise, hessian_rel=hrel,
+                               score_rel=srel)
         m.close()
 
 
@@ -181,6 +185,7 @@ def test_server_poisson_stream_full_map(full_wl, full_oracle, oracle_mod):
     alone = [m.align(s, q) for s, q in reqs]
     got = {}
     _print_case("stream: server 8 x 32")
+    gc.collect()  # no leftover matcher may free device memory while the server runs
     with m.server(slots=8, workgroups_per_slot=32, max_points=40000) as srv:
         tickets = []
         t_next = time.perf_counter()

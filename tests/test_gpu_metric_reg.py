@@ -8,7 +8,7 @@ import sys
 import numpy as np
 import pytest
 
-from tests import parity
+from oracle import parity
 
 pytestmark = pytest.mark.gpu
 

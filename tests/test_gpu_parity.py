@@ -10,7 +10,8 @@ import math
 import numpy as np
 import pytest
 
-from tests import npref, parity
+from tests import npref
+from oracle import parity
 
 pytestmark = pytest.mark.gpu
 
@@ -209,11 +210,12 @@ def test_fp32_derivatives_close_to_fp64_oracle(small, oracle_grid, oracle_mod):
 def test_fp32_alignment_matches_fp64_oracle(small, oracle_grid, oracle_mod):
     """Precision variant on the small workload: same status and iteration
     count as the FP64 oracle, pose within 1 mm / 0.01 deg (or, where the
-    oracle moves under FP32-size perturbations, within its probe spread)."""
+    oracle moves under FP32-size perturbations, the checks of
+    oracle/parity.py for such starts)."""
     _print_case("fp32 align")
     p = oracle_mod.params()
     r = _matcher(small.map_points, precision=1).align(small.scan, small.guess)
-    parity.check_alignment(r, oracle_grid, small.scan, small.guess, p, noise=1e-8, hessian_rel=1e-4)
+    parity.check_alignment(r, oracle_grid, small.scan, small.guess, p, noise=1e-6, hessian_rel=1e-4, score_rel=2e-5)
 
 
 @pytest.mark.parametrize("mode,ls", [(0, False), (1, False), (2, False), (3, False), (0, True)])

@@ -1,4 +1,11 @@
-"""Continuous-batching server: results alone and in the mix (GPU)."""
+"""Continuous-batching server: results alone and in the mix (GPU).
+
+A hipFree anywhere in the process waits for every launch on the device to
+end, the server's persistent launch included.  The tests therefore close
+their matchers explicitly and collect garbage before starting a server, so
+that no matcher left over from an earlier test is destroyed (and its buffers
+freed) while a server runs."""
+import gc
 import sys
 import time
 
@@ -52,12 +59,14 @@ def test_server_results_match_standalone(small):
     reqs = _requests(small)
     alone = [m.align(s, p) for s, p in reqs]
     got = {}
+    gc.collect()
     deadline = time.time() + 60
     with m.server(slots=3, workgroups_per_slot=W, max_points=2000) as srv:
         tickets = [srv.submit(s, p) for s, p in reqs[:4]]  # one more than the slots: it waits in the queue
         _drain(srv, got, 2, deadline)
         tickets += [srv.submit(s, p) for s, p in reqs[4:]]  # admitted while earlier ones run
         _drain(srv, got, len(reqs), deadline)
+    m.close()
     assert len(got) == len(reqs), "server did not finish every request"
     for t, a in zip(tickets, alone):
         r = got[t]
@@ -76,6 +85,7 @@ def test_server_errors(small):
     with pytest.raises(NdtError) as e:
         m.server(slots=4096, workgroups_per_slot=64, max_points=100)
     assert e.value.code == -7
+    gc.collect()
     with m.server(slots=2, workgroups_per_slot=4, max_points=100) as srv:
         with pytest.raises(NdtError) as e:
             srv.submit(small.scan[:101], small.guess)
@@ -86,6 +96,7 @@ def test_server_errors(small):
         t = srv.submit(small.scan[:100], small.guess)
         got = _drain(srv, {}, 1, time.time() + 30)
         assert t in got
+    m.close()
 
 
 def test_handle_refuses_other_work_while_serving(small):
@@ -98,6 +109,7 @@ def test_handle_refuses_other_work_while_serving(small):
     m = NdtScanMatcher(NdtScanMatcherConfig(workgroups_per_alignment=W))
     m.set_target(small.map_points)
     want = m.align(small.scan, small.guess)
+    gc.collect()
     srv = m.server(slots=2, workgroups_per_slot=W, max_points=small.scan.shape[0])
     try:
         with pytest.raises(NdtError):
@@ -114,6 +126,7 @@ def test_handle_refuses_other_work_while_serving(small):
     finally:
         srv.stop()
     r = m.align(small.scan, small.guess)
+    m.close()
     assert np.array_equal(r.pose_vector, want.pose_vector) and got[t].iterations == want.iterations
 
 
@@ -130,10 +143,13 @@ def test_server_idle_timeout_is_launch_wide(small):
     m.set_target(small.map_points)
     scan = np.ascontiguousarray(small.scan[:800])
     want = m.align(scan, small.guess)
+    gc.collect()
     srv = m.server(slots=2, workgroups_per_slot=W, max_points=scan.shape[0])
     try:
-        srv.set_idle_timeout(0.25)
-        t_end = time.time() + 1.5  # six timeouts' worth of serving on one slot
+        # A timeout of a few seconds: the host's own pauses between calls
+        # (scheduling, the interpreter) can reach a few hundred milliseconds.
+        srv.set_idle_timeout(2.0)
+        t_end = time.time() + 6.0  # three timeouts' worth of serving on one slot
         n_done = 0
         while time.time() < t_end:
             t = srv.submit(scan, small.guess)
@@ -143,7 +159,7 @@ def test_server_idle_timeout_is_launch_wide(small):
         burst = [srv.submit(scan, small.guess) for _ in range(2)]
         got = _drain(srv, {}, 2, time.time() + 30)
         assert set(burst) <= set(got)
-        time.sleep(1.0)  # no calls: the launch ends
+        time.sleep(3.0)  # no calls: the launch ends
         with pytest.raises(NdtError) as e:
             srv.submit(scan, small.guess)
         assert e.value.code == -5
@@ -151,4 +167,5 @@ def test_server_idle_timeout_is_launch_wide(small):
         srv.stop()
     assert n_done >= 3
     r = m.align(scan, small.guess)
+    m.close()
     assert np.array_equal(r.pose_vector, want.pose_vector)
