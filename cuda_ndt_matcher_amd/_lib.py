@@ -129,7 +129,7 @@ EXPORTS = [
     "ndt_hip_config_default", "ndt_hip_create", "ndt_hip_destroy", "ndt_hip_last_error",
     "ndt_hip_set_target", "ndt_hip_set_target_device", "ndt_hip_has_target", "ndt_hip_num_voxels",
     "ndt_hip_get_voxels", "ndt_hip_get_build_debug", "ndt_hip_lookup", "ndt_hip_derivatives",
-    "ndt_hip_align", "ndt_hip_align_device", "ndt_hip_align_batch", "ndt_hip_align_batch_device",
+    "ndt_hip_align", "ndt_hip_align_device", "ndt_hip_align_device_noalloc", "ndt_hip_align_batch", "ndt_hip_align_batch_device",
     "ndt_hip_align_scans", "ndt_hip_evaluate_nvtl", "ndt_hip_evaluate_transform_probability",
     "ndt_hip_per_point_scores", "ndt_hip_evaluate_nvtl_batch", "ndt_hip_set_regularization_pose",
     "ndt_hip_clear_regularization_pose", "ndt_hip_laplace_covariance_xy", "ndt_hip_laplace_covariance_6x6",
@@ -187,6 +187,7 @@ def load():
                                         C.POINTER(C.c_uint32)]
     lib.ndt_hip_align.argtypes = [vp, f32p, sz, f64p, rp]
     lib.ndt_hip_align_device.argtypes = [vp, vp, sz, f64p, rp]
+    lib.ndt_hip_align_device_noalloc.argtypes = [vp, vp, sz, f64p, vp]
     lib.ndt_hip_align_batch.argtypes = [vp, f32p, sz, f64p, sz, rp]
     lib.ndt_hip_align_batch_device.argtypes = [vp, vp, sz, f64p, sz, rp]
     lib.ndt_hip_align_scans.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(sz), f64p, sz, rp]

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -22,6 +23,33 @@
 #include "ndt_internal.hpp"
 
 using ndt::DevBuf;
+
+// Scratch of ndt_hip_align_device_noalloc: one alignment's job and exchange
+// state (ndt::kNoAllocScratchBytes, about 131 KiB), held by every GPU the
+// library runs on until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device array: the device address comes from
+// hipGetSymbolAddress, once per device, into g_scratch below.
+static __device__ __align__(256) unsigned char scratch[ndt::kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch[kMaxDevices] = {};
+
+// Looks up this device's copy of `scratch` (device current).  Once per device.
+hipError_t scratch_for_device(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[dev] = ptr;
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
+}  // namespace
 
 struct ndt_handle {
   ndt_config cfg{};
@@ -45,6 +73,7 @@ struct ndt_handle {
   double d1 = 0, d2 = 0;
   std::vector<int32_t> slot_to_index;  // voxel slot -> exported index (-1 if dropped)
   ndt_server* server = nullptr;        // the running continuous-batching server, if any
+  void* scratch = nullptr;             // this device's copy of the allocation-free scratch array
   ndt::SlotJob* host_jobs = nullptr;   // pinned staging for the jobs copy
   size_t host_jobs_cap = 0;
   // The last graph built for align* (use_graph) and what it was built for.
@@ -460,6 +489,10 @@ int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
   }
   h->coop_capacity = ndt::newton_max_coresident_workgroups();
   (void)hipDeviceGetAttribute(&h->wall_clock_khz, hipDeviceAttributeWallClockRate, h->device);
+  if (scratch_for_device(h->device, &h->scratch) != hipSuccess) {
+    ndt_hip_destroy(h);
+    return NDT_ERR_HIP;
+  }
   *out = h;
   return NDT_OK;
 }
@@ -657,6 +690,46 @@ int ndt_hip_align_device(ndt_handle* h, const float* d_xyz, size_t n, const doub
   if (!pose || !out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
   if (n == 0 || !d_xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
   return run_alignments(h, &d_xyz, &n, pose, 1, out);
+}
+
+int ndt_hip_align_device_noalloc(ndt_handle* h, const float* d_xyz, size_t n, const double pose[6],
+                                 ndt_result* d_out) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!pose || !d_out) return set_err(h, NDT_ERR_INVALID_ARGUMENT, "null argument");
+  if (n == 0 || !d_xyz) return set_err(h, NDT_ERR_EMPTY_INPUT, "Source point cloud is empty");
+  if (n >= (1ull << 31)) return set_err(h, NDT_ERR_CAPACITY, "scan exceeds 2^31 points");
+  // Same workgroup count as ndt_hip_align_device (configured or about one
+  // point per thread), without the autotuner, which launches extra trials.
+  const int W = choose_wg(h, n, 1);
+  if (W > ndt::kNoAllocMaxWorkgroups)
+    return set_err(h, NDT_ERR_CAPACITY, "allocation-free alignment supports at most 256 workgroups per alignment");
+  if (!h->scratch) return set_err(h, NDT_ERR_HIP, "allocation-free scratch is not set up on this device");
+  ndt::SlotJob job;
+  job.pts = d_xyz;
+  job.n = (uint32_t)n;
+  job.pad = 0;
+  for (int k = 0; k < 6; ++k) job.pose[k] = pose[k];
+  ndt::NewtonConfig cfg = newton_config(h, W);
+  cfg.phase = nullptr;  // phase stamps need a buffer of their own
+  auto* d_job = reinterpret_cast<ndt::SlotJob*>(h->scratch);
+  void* ws = reinterpret_cast<char*>(h->scratch) + ndt::kNoAllocWorkspaceOffset;
+  NDT_HIP(ndt::store_job(job, d_job, h->stream), "job store");
+  // launch_newton clears the arrival counters with a kernel before the
+  // Newton kernel; the partials are written before they are read.
+  if (hipError_t e = ndt::launch_newton(h->grid.view, cfg, d_job, 1, d_out, ws, h->stream, W > 1, h->ev0);
+      e != hipSuccess)
+    return launch_err(h, e, "newton launch", W > 1);
+  NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
+  NDT_HIP(hipStreamSynchronize(h->stream), "newton sync");
+  float ms = 0.f;
+  NDT_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time");
+  h->timing.align_ms = ms;
+  h->timing.align_workgroups = (uint32_t)W;
+  h->timing.align_slots = 1;
+  h->timing.align_graph = 0;
+  h->timing.align_tuned = 0;
+  return NDT_OK;
 }
 
 int ndt_hip_align_batch(ndt_handle* h, const float* xyz, size_t n, const double* poses, size_t m, ndt_result* out) {

@@ -156,6 +156,19 @@ hipError_t launch_server(const GridView& g, const NewtonConfig& cfg, ServerSlot*
                          hipStream_t st);
 int newton_max_coresident_workgroups();
 
+// The allocation-free single alignment (ndt_hip_align_device_noalloc) keeps
+// its job and exchange state in a fixed device array: one SlotJob at offset 0
+// and, from offset kNoAllocWorkspaceOffset, the workspace of one slot of at
+// most kNoAllocMaxWorkgroups workgroups (partials, then arrival counters).
+constexpr int kNoAllocMaxWorkgroups = 256;
+constexpr size_t kNoAllocWorkspaceOffset = 256;
+constexpr size_t kNoAllocWorkspaceBytes =
+    2ull * kNoAllocMaxWorkgroups * 32 * sizeof(double) + (1 + kNoAllocMaxWorkgroups / 16) * 32 * sizeof(uint32_t) + 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocWorkspaceOffset + kNoAllocWorkspaceBytes;
+// Writes job into d_job (device) with a one-thread kernel on st, so the job
+// needs no host staging buffer or runtime copy.
+hipError_t store_job(const SlotJob& job, SlotJob* d_job, hipStream_t st);
+
 // One derivative pass (score, gradient, Hessian, count) at a pose.
 hipError_t launch_derivatives(const GridView& g, const NewtonConfig& cfg, const float* d_pts, uint32_t n,
                               const double pose[6], int hess, double* d_partials, int blocks, double* d_out,

@@ -1048,6 +1048,8 @@ __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig
   }
 }
 
+__global__ void store_job_kernel(SlotJob job, SlotJob* __restrict__ dst) { *dst = job; }
+
 __global__ void zero_u32(uint32_t* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
@@ -1172,9 +1174,18 @@ __global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, co
 
 }  // namespace
 
-size_t newton_workspace_bytes(int slots, int wg_per_slot) {
-  const size_t partial = (size_t)slots * 2 * wg_per_slot * kPartialStride * sizeof(double);
-  return partial + (size_t)slots * counter_lines(wg_per_slot) * kCounterStride * sizeof(uint32_t) + 256;
+constexpr size_t workspace_bytes(int slots, int wg_per_slot) {
+  return (size_t)slots * 2 * wg_per_slot * kPartialStride * sizeof(double) +
+         (size_t)slots * counter_lines(wg_per_slot) * kCounterStride * sizeof(uint32_t) + 256;
+}
+static_assert(workspace_bytes(1, kNoAllocMaxWorkgroups) == kNoAllocWorkspaceBytes, "allocation-free workspace size");
+static_assert(sizeof(SlotJob) <= kNoAllocWorkspaceOffset, "allocation-free job slot");
+
+size_t newton_workspace_bytes(int slots, int wg_per_slot) { return workspace_bytes(slots, wg_per_slot); }
+
+hipError_t store_job(const SlotJob& job, SlotJob* d_job, hipStream_t st) {
+  hipLaunchKernelGGL(store_job_kernel, dim3(1), dim3(1), 0, st, job, d_job);
+  return hipGetLastError();
 }
 
 int newton_max_coresident_workgroups() {

@@ -198,6 +198,21 @@ int ndt_hip_derivatives(ndt_handle* h, const float* xyz, size_t n, const double 
 int ndt_hip_align(ndt_handle* h, const float* xyz, size_t n, const double pose[6], ndt_result* out);
 int ndt_hip_align_device(ndt_handle* h, const float* d_xyz, size_t n, const double pose[6], ndt_result* out);
 
+/* Allocation-free single alignment.  No counterpart in the reference: it is
+ * for callers that cannot allocate device or pinned memory during a call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same computation and result as ndt_hip_align_device with the
+ * same workgroup count (the autotuner and phase stamps are not used), but
+ * the result is written to d_out in device memory, and the job and exchange
+ * state live in a fixed device array of about 131 KiB per GPU, allocated
+ * with the library and held until the process ends: at most 256 workgroups
+ * per alignment (NDT_ERR_CAPACITY beyond).  The call allocates nothing and
+ * waits for the alignment on the handle's stream.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two threads, or two
+ * handles on different streams) race and must be serialised by the caller. */
+int ndt_hip_align_device_noalloc(ndt_handle* h, const float* d_xyz, size_t n, const double pose[6],
+                                 ndt_result* d_out);
+
 /* align_batch / align_batch_gpu (ndt.rs:903-1001): one scan, m initial poses,
  * all in one persistent launch. */
 int ndt_hip_align_batch(ndt_handle* h, const float* xyz, size_t n, const double* poses, size_t m, ndt_result* out);
