@@ -764,6 +764,7 @@ def main():
             wall.append(1e3 * (time.perf_counter() - t1))
             kern.append(m.timing()["align_ms"])
             by_seed[j] = rr
+    phases = m.phase_profile() if args.phases else None  # the last trial start's alignment
     statuses = {}
     for rr in by_seed:
         statuses[rr.status] = statuses.get(rr.status, 0) + 1
@@ -779,7 +780,7 @@ def main():
 
     extra = {}
     if args.phases:
-        extra["phases_us"] = m.phase_profile()
+        extra["phases_us"] = phases
         print(f"[bench] phases {extra['phases_us']}", file=sys.stderr, flush=True)
     if args.particles:
         # Initial-pose search: particles around the true pose (translation

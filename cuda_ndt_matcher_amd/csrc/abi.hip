@@ -221,9 +221,13 @@ int bit_length(size_t v) {
 // the cached graph, rebuilding it when the shape, a buffer or the
 // configuration changed.
 int launch_once(ndt_handle* h, size_t m, int W, bool allow_graph, float* ms, bool* used_graph) {
-  NDT_HIP(h->ws.reserve(ndt::newton_workspace_bytes((int)m, W)), "alloc workspace");
+  // Several workgroups per alignment exchange partials through write-once
+  // buffers, one per exchange the alignment can make.
+  const int E = W > 1 ? ndt::newton_exchanges(h->cfg.max_iterations, h->cfg.use_line_search != 0) : 0;
+  NDT_HIP(h->ws.reserve(ndt::newton_workspace_bytes((int)m, W, E)), "alloc workspace");
   if (h->cfg.profile_phases) NDT_HIP(h->phase.reserve(16 * sizeof(unsigned long long)), "alloc phase");
-  const ndt::NewtonConfig cfg = newton_config(h, W);  // after the reserves: it carries their pointers
+  ndt::NewtonConfig cfg = newton_config(h, W);  // after the reserves: it carries their pointers
+  cfg.exchanges = E;
   const bool graph = allow_graph && h->cfg.use_graph && !h->cfg.profile_phases;
   *used_graph = graph;
   if (graph) {

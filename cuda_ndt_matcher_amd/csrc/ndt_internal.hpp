@@ -92,6 +92,10 @@ struct NewtonConfig {
   int32_t reg_active;  // regularisation enabled and a reference pose set
   int32_t precision;
   int32_t wg_per_slot;
+  // Slot exchange buffers written once per launch (0 = two alternating
+  // buffers with arrival counters; see slot_reduce in newton.hip).
+  int32_t exchanges;
+  int32_t pad0;
   // Phase stamps (null = off): workgroup 0 accumulates wall-clock ticks per
   // phase into phase[0..5] and counts into phase[6..7].
   unsigned long long* phase;
@@ -103,7 +107,13 @@ static_assert(sizeof(ndt_result) == 6 * 8 + 8 + 3 * 8 + 16 + 36 * 8, "ndt_result
 // writes ndt_result per slot (device array).  ws must hold
 // newton_workspace_bytes(slots, wg_per_slot).  ev_start (optional) is
 // recorded after the counter reset, just before the Newton kernel.
-size_t newton_workspace_bytes(int slots, int wg_per_slot);
+size_t newton_workspace_bytes(int slots, int wg_per_slot, int exchanges = 0);
+// Exchanges one alignment of at most max_iter iterations can make: one per
+// derivative pass with the Hessian, up to 11 per iteration for the
+// More-Thuente trials, one for the final NVTL.
+inline int newton_exchanges(uint32_t max_iter, bool line_search) {
+  return (int)max_iter * (line_search ? 12 : 1) + 1;
+}
 hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJob* d_jobs, int slots,
                          ndt_result* d_out, void* ws, hipStream_t st, bool cooperative, hipEvent_t ev_start);
 // The launch_newton sequence as an instantiated graph: jobs copied from

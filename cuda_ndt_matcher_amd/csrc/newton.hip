@@ -452,7 +452,16 @@ struct SlotState {
 
 // Slot-wide exchange state, identical in every workgroup of a slot.
 //
-// Arrivals are counted in two levels: each lane that stores a partial value
+// Two protocols.  Write-once (exchanges > 0, the one-shot launches): exchange
+// e of a launch uses its own buffer, filled with kExchangeEmpty before the
+// launch; each workgroup stores its partials there with relaxed agent-scope
+// atomic stores and every workgroup reads all of them with relaxed atomic
+// loads, reloading a value until it is no longer the sentinel.  A location
+// is written once per launch, so a value that is not the sentinel is the
+// exchange's own and no fence or arrival counter is needed.
+//
+// Alternating buffers with arrival counters (exchanges == 0: the server,
+// whose slots run any number of jobs).  Arrivals are counted in two levels: each lane that stores a partial value
 // adds to the counter of its workgroup's group of kArrivalGroup ranks (its
 // own release covers its own store), and the lane that completes a group
 // adds 1 to the slot's top counter, which every workgroup polls.  (One
@@ -460,8 +469,13 @@ struct SlotState {
 // Counters sit in separate 128-byte lines.
 constexpr int kArrivalGroup = 16;
 constexpr int kCounterStride = 32;  // uint32 per counter line
+// Bits of the sentinel that marks an exchange value not yet written: a
+// signalling NaN, which no arithmetic produces.
+constexpr unsigned long long kExchangeEmpty = 0x7FF0DEADBEEF0001ull;
+
 struct SlotSync {
-  double* partials;         // 2 x W x kPartialStride doubles (alternating buffers)
+  double* partials;         // E x W x kPartialStride doubles (write-once) or 2 x W x kPartialStride (alternating)
+  int E;                    // write-once buffers per slot (0: alternating protocol)
   uint32_t* top;            // slot counter: completed groups
   uint32_t* group;          // this rank's group counter: arrivals
   uint32_t group_size;      // workgroups in this rank's group
@@ -491,6 +505,64 @@ __device__ inline void slot_reduce(double (&v)[N], int count, double* lds, doubl
   const int W = sy.W;
   if (W == 1) {
     if (t < count) tot[t] = wg;
+    __syncthreads();
+    return;
+  }
+  if (sy.E > 0) {
+    // Write-once buffer of this exchange (the host sizes E for every
+    // exchange of an alignment; the modulo only keeps the index in bounds).
+    unsigned long long* wb =
+        reinterpret_cast<unsigned long long*>(sy.partials + (size_t)(sy.gen % (uint32_t)sy.E) * W * kPartialStride);
+    ++sy.gen;
+    if (t < count)
+      __hip_atomic_store(wb + (size_t)sy.rank * kPartialStride + t, (unsigned long long)__double_as_longlong(wg),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Same reading order and summation as the alternating protocol below.
+    const int pair = t & 15, grp = t >> 4;
+    double s0 = 0.0, s1 = 0.0;
+    if (2 * pair < count) {
+      const bool second = 2 * pair + 1 < count;
+      const unsigned long long* col = wb + 2 * pair;
+      for (int r0 = grp; r0 < W; r0 += 8 * kRankGroups) {
+        unsigned long long x[8][2];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int r = r0 + k * kRankGroups;
+          const unsigned long long* c = col + (size_t)r * kPartialStride;
+          x[k][0] = r < W ? __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+          x[k][1] = (r < W && second) ? __hip_atomic_load(c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int r = r0 + k * kRankGroups;
+          const unsigned long long* c = col + (size_t)r * kPartialStride;
+          // Every workgroup of the slot is resident (cooperative launch) and
+          // stores its values in this exchange, so each wait ends.
+          while (x[k][0] == kExchangeEmpty) {
+            __builtin_amdgcn_s_sleep(1);
+            x[k][0] = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          while (x[k][1] == kExchangeEmpty) {
+            __builtin_amdgcn_s_sleep(1);
+            x[k][1] = __hip_atomic_load(c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          s0 += __longlong_as_double((long long)x[k][0]);
+          s1 += __longlong_as_double((long long)x[k][1]);
+        }
+      }
+    }
+    lds[grp * kPartialStride + 2 * pair] = s0;
+    lds[grp * kPartialStride + 2 * pair + 1] = s1;
+    __syncthreads();
+    if (t < count) {
+      double s = 0.0;
+#pragma unroll
+      for (int g = 0; g < kRankGroups; ++g) s += lds[g * kPartialStride + t];
+      tot[t] = s;
+    }
     __syncthreads();
     return;
   }
@@ -611,9 +683,10 @@ __device__ inline void slot_range(uint32_t n, int W, int rank, uint32_t& begin, 
 }
 
 // Exchange state of workgroup `rank` of slot `slot`.
-__device__ inline SlotSync make_slot_sync(double* partials, uint32_t* counters, int slot, int W, int rank) {
+__device__ inline SlotSync make_slot_sync(double* partials, uint32_t* counters, int slot, int W, int rank, int E) {
   SlotSync sy;
-  sy.partials = partials + (size_t)slot * 2 * W * kPartialStride;
+  sy.E = E;
+  sy.partials = partials + (size_t)slot * (E > 0 ? E : 2) * W * kPartialStride;
   uint32_t* lines = counters + (size_t)slot * counter_lines(W) * kCounterStride;
   sy.top = lines;
   sy.group = lines + (size_t)(1 + rank / kArrivalGroup) * kCounterStride;
@@ -925,7 +998,7 @@ __global__ __launch_bounds__(kBlock) void newton_kernel(GridView g, NewtonConfig
                                                         uint32_t* __restrict__ counters) {
   const int W = cfg.wg_per_slot;
   const int slot = blockIdx.x / W;
-  SlotSync sy = make_slot_sync(partials, counters, slot, W, blockIdx.x % W);
+  SlotSync sy = make_slot_sync(partials, counters, slot, W, blockIdx.x % W, cfg.exchanges);
   const SlotJob job = jobs[slot];
   align_slot<kF32>(g, cfg, job, out + slot, sy);
 }
@@ -994,7 +1067,7 @@ __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig
   const int W = cfg.wg_per_slot;
   const int slot = blockIdx.x / W;
   const int rank = blockIdx.x % W;
-  SlotSync sy = make_slot_sync(partials, counters, slot, W, rank);
+  SlotSync sy = make_slot_sync(partials, counters, slot, W, rank, 0);
   unsigned long long* my_cmd = cmd + (size_t)slot * kCounterStride / 2;
   // One line after the command words: jobs finished by the launch.
   unsigned long long* activity = cmd + (size_t)gridDim.x / W * kCounterStride / 2;
@@ -1049,6 +1122,10 @@ __global__ __launch_bounds__(kBlock) void server_kernel(GridView g, NewtonConfig
 }
 
 __global__ void store_job_kernel(SlotJob job, SlotJob* __restrict__ dst) { *dst = job; }
+
+__global__ void fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
 
 __global__ void zero_u32(uint32_t* p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
@@ -1174,14 +1251,21 @@ __global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, co
 
 }  // namespace
 
-constexpr size_t workspace_bytes(int slots, int wg_per_slot) {
-  return (size_t)slots * 2 * wg_per_slot * kPartialStride * sizeof(double) +
+// Workspace layout: partials (slots x (E or 2) x W x kPartialStride doubles),
+// then the arrival counters of the alternating protocol.
+constexpr size_t partials_bytes(int slots, int wg_per_slot, int exchanges) {
+  return (size_t)slots * (exchanges > 0 ? exchanges : 2) * wg_per_slot * kPartialStride * sizeof(double);
+}
+constexpr size_t workspace_bytes(int slots, int wg_per_slot, int exchanges) {
+  return partials_bytes(slots, wg_per_slot, exchanges) +
          (size_t)slots * counter_lines(wg_per_slot) * kCounterStride * sizeof(uint32_t) + 256;
 }
-static_assert(workspace_bytes(1, kNoAllocMaxWorkgroups) == kNoAllocWorkspaceBytes, "allocation-free workspace size");
+static_assert(workspace_bytes(1, kNoAllocMaxWorkgroups, 0) == kNoAllocWorkspaceBytes, "allocation-free workspace size");
 static_assert(sizeof(SlotJob) <= kNoAllocWorkspaceOffset, "allocation-free job slot");
 
-size_t newton_workspace_bytes(int slots, int wg_per_slot) { return workspace_bytes(slots, wg_per_slot); }
+size_t newton_workspace_bytes(int slots, int wg_per_slot, int exchanges) {
+  return workspace_bytes(slots, wg_per_slot, exchanges);
+}
 
 hipError_t store_job(const SlotJob& job, SlotJob* d_job, hipStream_t st) {
   hipLaunchKernelGGL(store_job_kernel, dim3(1), dim3(1), 0, st, job, d_job);
@@ -1212,9 +1296,15 @@ hipError_t launch_newton(const GridView& g, const NewtonConfig& cfg, const SlotJ
   const int W = cfg.wg_per_slot;
   double* partials = reinterpret_cast<double*>(ws);
   uint32_t* counters =
-      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
-  const int ncounters = slots * counter_lines(W) * kCounterStride;
-  hipLaunchKernelGGL(zero_u32, dim3(std::min(1024, (ncounters + 255) / 256)), dim3(256), 0, st, counters, ncounters);
+      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + partials_bytes(slots, W, cfg.exchanges));
+  if (cfg.exchanges > 0) {
+    const size_t nfill = partials_bytes(slots, W, cfg.exchanges) / sizeof(double);
+    hipLaunchKernelGGL(fill_u64, dim3((unsigned)std::min<size_t>(2048, (nfill + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned long long*>(partials), nfill, kExchangeEmpty);
+  } else {
+    const int ncounters = slots * counter_lines(W) * kCounterStride;
+    hipLaunchKernelGGL(zero_u32, dim3(std::min(1024, (ncounters + 255) / 256)), dim3(256), 0, st, counters, ncounters);
+  }
   if (ev_start) {
     const hipError_t e = hipEventRecord(ev_start, st);
     if (e != hipSuccess) return e;
@@ -1250,8 +1340,11 @@ hipError_t build_newton_graph(const GridView& g, const NewtonConfig& cfg, SlotJo
   const int W = cfg.wg_per_slot;
   double* partials = reinterpret_cast<double*>(ws);
   uint32_t* counters =
-      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + (size_t)slots * 2 * W * kPartialStride * sizeof(double));
+      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ws) + partials_bytes(slots, W, cfg.exchanges));
   int ncounters = slots * counter_lines(W) * kCounterStride;
+  unsigned long long* fill_ptr = reinterpret_cast<unsigned long long*>(partials);
+  size_t nfill = partials_bytes(slots, W, cfg.exchanges) / sizeof(double);
+  unsigned long long fill_val = kExchangeEmpty;
   hipGraph_t graph = nullptr;
   hipError_t e = hipGraphCreate(&graph, 0);
   if (e != hipSuccess) return e;
@@ -1259,13 +1352,22 @@ hipError_t build_newton_graph(const GridView& g, const NewtonConfig& cfg, SlotJo
   e = hipGraphAddMemcpyNode1D(&n_jobs, graph, nullptr, 0, d_jobs, h_jobs, (size_t)slots * sizeof(SlotJob),
                               hipMemcpyHostToDevice);
   if (e == hipSuccess) {
+    // Exchange buffers to the sentinel (write-once protocol) or counters to
+    // zero, by a kernel node: no memset nodes.
     hipKernelNodeParams kp{};
     void* zargs[] = {&counters, &ncounters};
-    kp.func = reinterpret_cast<void*>(zero_u32);
-    kp.gridDim = dim3(std::min(1024, (ncounters + 255) / 256));
+    void* fargs[] = {&fill_ptr, &nfill, &fill_val};
+    if (cfg.exchanges > 0) {
+      kp.func = reinterpret_cast<void*>(fill_u64);
+      kp.gridDim = dim3((unsigned)std::min<size_t>(2048, (nfill + 255) / 256));
+      kp.kernelParams = fargs;
+    } else {
+      kp.func = reinterpret_cast<void*>(zero_u32);
+      kp.gridDim = dim3(std::min(1024, (ncounters + 255) / 256));
+      kp.kernelParams = zargs;
+    }
     kp.blockDim = dim3(256);
     kp.sharedMemBytes = 0;
-    kp.kernelParams = zargs;
     kp.extra = nullptr;
     e = hipGraphAddKernelNode(&n_zero, graph, &n_jobs, 1, &kp);
   }
