@@ -373,6 +373,114 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     return out
 
 
+def init_pose_bench(seed, train_maps=8, per_map=4096, epochs=150, n_test=32, with_cpu=True, train_beams=128):
+    """The init-pose half of the metric on the 5M-point map with ~30k-point
+    scans: solves/s at <= 2 cm / 0.1 deg of the alignment started at the true
+    pose.  Each request is a 128-beam scan at a new pose of the 1 x 1 km scene
+    with a position prior U(+-1 m) off and no heading.  Searches:
+      - plain: 200 particles around the prior (translation U(+-2 m), yaw
+        U(+-pi), roll/pitch N(0, 1 deg)), one launch, best NVTL;
+      - plain with the prior's roll/pitch (0) instead of N(0, 1 deg);
+      - plain, 1000 particles and a polish launch of the 4 best;
+      - plain, 16 particles with the prior's roll/pitch and a polish launch:
+        the learned search's budget without the network;
+      - learned: the bf16 MLP's 16 prior-relative hypotheses refined in one
+        launch (roll/pitch from the prior), a polish launch, and below NVTL
+        2.3 the plain 200-particle search with the prior's roll/pitch as
+        fallback.
+        The network is trained here on fresh 200 x 200 m procedural scenes
+        (train_maps of them, scans of the same 128-beam lidar at 900
+        azimuth steps), never on this map.
+    Every search's time includes its upload (and, for the learned one, the
+    descriptor and the network)."""
+    import torch
+
+    from cuda_ndt_matcher_amd import NdtScanMatcher, NdtScanMatcherConfig, prior_proposal as pp, proposal, synth
+
+    t0 = time.perf_counter()
+    data = pp.TrainingSet()
+    for i in range(train_maps):
+        data.add_scene(1000 + 16 * seed + i, seed * 7919 + 500 + i, per_map, beams=train_beams, az_steps=900,
+                       matcher_factory=lambda: NdtScanMatcher(NdtScanMatcherConfig()))
+    data.finish()
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = proposal.ProposalNet(proposal.NET_SIZES, seed=seed)
+    t1 = time.perf_counter()
+    losses = pp.train_relative(net, data, epochs, seed=seed)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    sizes = proposal.NET_SIZES
+    mac = sum(a * b for a, b in zip(sizes[:-1], sizes[1:]))
+    steps = epochs * (len(data) // 256)
+    train_tflops = 6.0 * 256 * mac * steps / t_train / 1e12  # forward + both backward products per step
+    n_train = len(data)
+    del data
+
+    wl = synth.make_workload("full", seed)
+    extent, nb, npl = synth.WORKLOADS["full"][:3]
+    scene = synth.Scene(seed, extent, nb, npl)
+    reqs = pp.scene_requests(scene, seed * 7919 + 999983, n_test, 0.3 * extent, 128, 1800, 0.3)
+    m = NdtScanMatcher(NdtScanMatcherConfig())
+    m.set_target(wl.map_points)
+    refs = pp.reference_poses(m, reqs, chunk=8)
+    quality = pp.hypothesis_quality(net, reqs, refs)
+    dev = pp.DeviceRequest()
+    res = {"plain_200": [0, 0.0], "plain_200_prior_roll_pitch": [0, 0.0], "plain_16_prior_roll_pitch": [0, 0.0],
+           "plain_1000_polish": [0, 0.0], "learned": [0, 0.0]}
+    fallbacks, stage = 0, {}
+    for k, (q, ref) in enumerate(zip(reqs, refs)):
+        for name, parts, pol, rp in (("plain_200", 200, 0, 1.0), ("plain_200_prior_roll_pitch", 200, 0, 0.0),
+                                     ("plain_16_prior_roll_pitch", 16, 4, 0.0), ("plain_1000_polish", 1000, 4, 1.0)):
+            t = time.perf_counter()
+            pn = dev.align.load_points(q.scan)
+            r = pp.plain_prior_search(m, q, parts, seed + k, polish=pol, roll_pitch_deg=rp, ptr_n=pn)
+            res[name][1] += time.perf_counter() - t
+            res[name][0] += pp.success(r.pose_vector, ref)
+        t = time.perf_counter()
+        r, fb = pp.relative_search(m, net, q, dev, times=stage,
+                                   fallback=lambda q_, k=k: pp.plain_prior_search(m, q_, 200, seed + k,
+                                                                                   roll_pitch_deg=0.0))
+        res["learned"][1] += time.perf_counter() - t
+        res["learned"][0] += pp.success(r.pose_vector, ref)
+        fallbacks += fb
+    ref_nvtl = [r.nvtl for r in m.align_parallel_scans([(q.scan, p) for q, p in zip(reqs[:8], refs[:8])])]
+    out = {"map": "5M-point 1 x 1 km map (not in training)", "requests": n_test,
+           "reference_nvtl_p50": float(np.median(ref_nvtl)),
+           "scan_points_median": int(np.median([q.scan.shape[0] for q in reqs])),
+           "prior": "position U(+-1 m) on x, y; no heading",
+           "train": {"scenes": train_maps, "beams": train_beams, "scans": n_train, "epochs": epochs, "data_s": t_data, "train_s": t_train,
+                     "loss_first_last": [losses[0], losses[-1]],
+                     "mlp_bf16_tflops": train_tflops, "mlp_bf16_frac": train_tflops / 2500.0}}
+    for name, (ok, el) in res.items():
+        out[name] = {"success_rate": ok / n_test, "solves_per_s": n_test / el}
+    out["learned"]["fallback_rate"] = fallbacks / n_test
+    out["learned"]["stage_ms_per_solve"] = {key: 1e3 * v / n_test for key, v in stage.items()}
+    fwd_s = stage.get("proposal", 0.0) / n_test
+    out["learned"]["mlp_forward"] = {"gflop": 2.0 * 32 * mac / 1e9, "ms": 1e3 * fwd_s,
+                                     "tflops": (2.0 * 32 * mac / fwd_s / 1e12) if fwd_s > 0 else None}
+    out["best_hypothesis_m_deg_p50"] = [float(np.median(quality[:, 0])), float(np.median(quality[:, 1]))]
+    out["hypothesis_within_1m_5deg"] = int(np.sum((quality[:, 0] <= 1.0) & (quality[:, 1] <= 5.0)))
+    out["speedup_vs_plain_200"] = out["learned"]["solves_per_s"] / out["plain_200"]["solves_per_s"]
+    out["speedup_vs_plain_1000_polish"] = out["learned"]["solves_per_s"] / out["plain_1000_polish"]["solves_per_s"]
+    if with_cpu:
+        cases = []
+        for k, q in enumerate(reqs[:4]):
+            r = np.random.default_rng(seed + k)
+            cases.append((q.scan, [np.array([q.prior[0] + r.uniform(-2, 2), q.prior[1] + r.uniform(-2, 2), q.prior[2],
+                                             r.normal(0, math.radians(1)), r.normal(0, math.radians(1)),
+                                             r.uniform(-math.pi, math.pi)]) for _ in range(200)]))
+        out["cpu_baseline"] = cpu_particle_baseline(wl.map_points, cases)
+    m.close()
+    net.close()
+    print(f"[bench] init pose ({n_test} requests, 5M-point map, network trained on {train_maps} other scenes): "
+          + ", ".join(f"{k} {out[k]['success_rate'] * 100:.0f}% at {out[k]['solves_per_s']:.1f} solves/s"
+                      for k in res) + f"; learned fallbacks {fallbacks}; best hypothesis p50 "
+          f"{out['best_hypothesis_m_deg_p50'][0]:.2f} m / {out['best_hypothesis_m_deg_p50'][1]:.1f} deg; "
+          f"training {t_train:.1f} s at {train_tflops:.0f} TFLOP/s", file=sys.stderr, flush=True)
+    return out
+
+
 def scoring_bench(m, wl, ptr, n, seed, with_cpu=True, reps=10):
     """Batched scoring on the 30k-point scan and the 5M-point map (device
     resident scan): evaluate_nvtl_batch over 200 poses around the true pose
@@ -693,6 +801,8 @@ def main():
                     help="also time batched NVTL, TP and per-point scoring on the 30k-point scan")
     ap.add_argument("--ingest", action="store_true",
                     help="also time the sensor point filter and alignments during a background map rebuild")
+    ap.add_argument("--no-init-pose", action="store_true",
+                    help="skip the init-pose searches (learned proposal vs plain particle search)")
     ap.add_argument("--batching", action="store_true",
                     help="also compare direct launches, graph replay and the autotuner on three launch shapes")
     args = ap.parse_args()
@@ -805,7 +915,7 @@ def main():
         err_r = float(np.rad2deg(np.abs(bp[3:] - ref[3:]).max()))
         ok = sum(1 for x in res if np.abs(x.pose_vector[:3] - ref[:3]).max() <= 0.02
                  and np.rad2deg(np.abs(x.pose_vector[3:] - ref[3:]).max()) <= 0.1)
-        extra["init_pose"] = {"particles": args.particles, "ms_per_solve": 1e3 * bt, "solves_per_s": 1.0 / bt,
+        extra["particle_search"] = {"particles": args.particles, "ms_per_solve": 1e3 * bt, "solves_per_s": 1.0 / bt,
                               "particle_aligns_per_s": args.particles / bt, "best_nvtl": best.nvtl,
                               "best_err_m": err_t, "best_err_deg": err_r,
                               "success": bool(err_t <= 0.02 and err_r <= 0.1),
@@ -813,7 +923,7 @@ def main():
         print(f"[bench] init pose: {args.particles} particles in {1e3 * bt:.2f} ms, best err {100 * err_t:.2f} cm "
               f"{err_r:.3f} deg, {ok} particles within 2 cm", file=sys.stderr, flush=True)
         if not args.no_cpu:
-            extra["init_pose"]["cpu_baseline"] = cpu_particle_baseline(wl.map_points, [(wl.scan, parts)])
+            extra["particle_search"]["cpu_baseline"] = cpu_particle_baseline(wl.map_points, [(wl.scan, parts)])
             print(f"[bench] init pose CPU: {extra['init_pose']['cpu_baseline']['value']:.3f} solves/s on "
                   f"{extra['init_pose']['cpu_baseline']['cores']} threads", file=sys.stderr, flush=True)
 
@@ -860,6 +970,8 @@ def main():
         extra["scoring"] = scoring_bench(m, wl, ptr, n, args.seed, with_cpu=not args.no_cpu)
     if args.precision_sweep:
         extra["precision_sweep"] = precision_sweep(args.seed, max(3, args.steps // 5))
+    if not args.no_init_pose:
+        extra["init_pose"] = init_pose_bench(args.seed, with_cpu=not args.no_cpu)
     line.update(extra)
     if not args.no_cpu:
         from oracle import oracle as O

@@ -195,6 +195,20 @@ class QueryScan:
         self._h = torch.empty((self._cap, 3), dtype=torch.float32).pin_memory()
         self.points = torch.empty((self._cap, 3), dtype=torch.float32, device="cuda:0")
 
+    def load_points(self, scan: np.ndarray):
+        """Uploads `scan` only (no descriptor); returns (device pointer, n)
+        once the copy is complete."""
+        import torch
+
+        a = np.ascontiguousarray(np.asarray(scan, np.float32).reshape(-1, 3))
+        n = a.shape[0]
+        if n > self._cap:
+            self._grow(n)
+        self._h[:n].numpy()[...] = a
+        self.points[:n].copy_(self._h[:n], non_blocking=True)
+        torch.cuda.synchronize()
+        return self.points.data_ptr(), n
+
     def load(self, scan: np.ndarray):
         """Uploads `scan` and forms its descriptor; returns (device pointer, n)."""
         import torch
