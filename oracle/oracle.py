@@ -14,7 +14,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libndt_oracle.so")
+# NDT_ORACLE_LIB selects another build of the same source (the
+# AddressSanitizer/UBSan build that `make -C oracle sanitize` writes).
+_LIB_PATH = os.environ.get("NDT_ORACLE_LIB") or os.path.join(_HERE, "libndt_oracle.so")
 
 SEARCH_KDTREE, SEARCH_DIRECT26, SEARCH_DIRECT7, SEARCH_DIRECT1, SEARCH_CELL27 = 0, 1, 2, 3, 4
 
