@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace ndt {
 
@@ -180,19 +181,20 @@ __device__ inline uint32_t box_index(const GridView& g, int32_t x, int32_t y, in
 }
 
 // Cell-table visit of the candidate cells around the cell with box index
-// `lin`, eight at a time: their 16-byte entries (mean, slot) are loaded
-// together, then the records of the accepted ones.  The gather rate of each
-// CU bounds these lookups, so the rejected candidates cost one load each.
+// `lin`, sixteen at a time: their 16-byte entries (mean, slot) are loaded
+// together, then the records of the accepted ones, eight at a time (a
+// second group only when a lane accepted a candidate past its eighth).
 // Candidates are visited in increasing cell order, as in probe_cells.
 template <class A, class F>
 __device__ inline void probe_dense(const GridView& g, uint32_t lin, uint32_t cand, A&& accept, F&& visit) {
+  constexpr int kEnt = 12, kRec = 6;
   const int32_t sy = (int32_t)g.bdim[0], sz = (int32_t)(g.bdim[0] * g.bdim[1]);
 #pragma unroll 1
   while (cand) {
-    float4 ent[8];
-    bool ok[8];
+    float4 ent[kEnt];
+    bool ok[kEnt];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < kEnt; ++b) {
       ok[b] = false;
       if (cand) {
         const int c = __ffs(cand) - 1;
@@ -203,19 +205,29 @@ __device__ inline void probe_dense(const GridView& g, uint32_t lin, uint32_t can
       }
     }
 #pragma unroll
-    for (int b = 0; b < 8; ++b) ok[b] = ok[b] && accept(ent[b].x, ent[b].y, ent[b].z);
-    float4 m0[8], m1[8], m2[8];
+    for (int b = 0; b < kEnt; ++b) ok[b] = ok[b] && accept(ent[b].x, ent[b].y, ent[b].z);
+    uint32_t slot[kEnt];
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
-      if (ok[b]) {
-        const uint32_t sl = __float_as_uint(ent[b].w);
-        m0[b] = g.vox[3 * sl];
-        m1[b] = g.vox[3 * sl + 1];
-        m2[b] = g.vox[3 * sl + 2];
-      }
+    for (int b = 0; b < kEnt; ++b) slot[b] = __float_as_uint(ent[b].w);
+    auto records = [&](auto h) {
+      constexpr int o = decltype(h)::value * kRec;
+      float4 m0[kRec], m1[kRec], m2[kRec];
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
-      if (ok[b]) visit((int32_t)__float_as_uint(ent[b].w), VoxelRecord{m0[b], m1[b], m2[b]});
+      for (int b = 0; b < kRec; ++b)
+        if (ok[o + b]) {
+          m0[b] = g.vox[3 * slot[o + b]];
+          m1[b] = g.vox[3 * slot[o + b] + 1];
+          m2[b] = g.vox[3 * slot[o + b] + 2];
+        }
+#pragma unroll
+      for (int b = 0; b < kRec; ++b)
+        if (ok[o + b]) visit((int32_t)slot[o + b], VoxelRecord{m0[b], m1[b], m2[b]});
+    };
+    records(std::integral_constant<int, 0>{});
+    bool late = false;
+#pragma unroll
+    for (int b = kRec; b < kEnt; ++b) late |= ok[b];
+    if (__any(late)) records(std::integral_constant<int, 1>{});
   }
 }
 
