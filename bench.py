@@ -373,7 +373,8 @@ def learned_bench(seed, n_train=262144, n_test=64, epochs=60, half=60.0, with_cp
     return out
 
 
-def init_pose_bench(seed, train_maps=8, per_map=4096, epochs=150, n_test=32, with_cpu=True, train_beams=128):
+def init_pose_bench(seed, train_maps=64, per_map=4096, epochs=50, n_test=32, with_cpu=True, train_beams=128,
+                    train_az=225):
     """The init-pose half of the metric on the 5M-point map with ~30k-point
     scans: solves/s at <= 2 cm / 0.1 deg of the alignment started at the true
     pose.  Each request is a 128-beam scan at a new pose of the 1 x 1 km scene
@@ -389,8 +390,10 @@ def init_pose_bench(seed, train_maps=8, per_map=4096, epochs=150, n_test=32, wit
         2.3 the plain 200-particle search with the prior's roll/pitch as
         fallback.
         The network is trained here on fresh 200 x 200 m procedural scenes
-        (train_maps of them, scans of the same 128-beam lidar at 900
-        azimuth steps), never on this map.
+        (train_maps of them, scans of the same 128-beam lidar at train_az
+        azimuth steps: many scenes at a coarse azimuth step generalise
+        better than few at a fine one for the same ray-casting time), never
+        on this map.
     Every search's time includes its upload (and, for the learned one, the
     descriptor and the network)."""
     import torch
@@ -400,7 +403,7 @@ def init_pose_bench(seed, train_maps=8, per_map=4096, epochs=150, n_test=32, wit
     t0 = time.perf_counter()
     data = pp.TrainingSet()
     for i in range(train_maps):
-        data.add_scene(1000 + 16 * seed + i, seed * 7919 + 500 + i, per_map, beams=train_beams, az_steps=900,
+        data.add_scene(1000 + 16 * seed + i, seed * 7919 + 500 + i, per_map, beams=train_beams, az_steps=train_az,
                        matcher_factory=lambda: NdtScanMatcher(NdtScanMatcherConfig()))
     data.finish()
     torch.cuda.synchronize()
@@ -449,7 +452,7 @@ def init_pose_bench(seed, train_maps=8, per_map=4096, epochs=150, n_test=32, wit
            "reference_nvtl_p50": float(np.median(ref_nvtl)),
            "scan_points_median": int(np.median([q.scan.shape[0] for q in reqs])),
            "prior": "position U(+-1 m) on x, y; no heading",
-           "train": {"scenes": train_maps, "beams": train_beams, "scans": n_train, "epochs": epochs, "data_s": t_data, "train_s": t_train,
+           "train": {"scenes": train_maps, "beams": train_beams, "azimuth_steps": train_az, "scans": n_train, "epochs": epochs, "data_s": t_data, "train_s": t_train,
                      "loss_first_last": [losses[0], losses[-1]],
                      "mlp_bf16_tflops": train_tflops, "mlp_bf16_frac": train_tflops / 2500.0}}
     for name, (ok, el) in res.items():
