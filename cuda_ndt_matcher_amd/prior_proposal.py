@@ -175,12 +175,15 @@ def hypotheses(net: ProposalNet, batch_ptr: int, k: int = HYPOTHESES) -> np.ndar
 
 def relative_search(matcher, net: ProposalNet, req: Request, dev: DeviceRequest, k: int = HYPOTHESES,
                     polish: int = 4, nvtl_threshold: float = NVTL_THRESHOLD, fallback=None,
-                    times: Optional[dict] = None):
+                    times: Optional[dict] = None, quarter_turns: int = 8):
     """Hypotheses -> one batched refinement (height from the prior, roll and
     pitch zero) -> when the best refined pose has not converged, the
-    `polish` best once more -> fallback(req) when the best NVTL is below the
-    threshold.  `times` accumulates seconds per stage.  Returns (best
-    AlignResult, used_fallback)."""
+    `polish` best once more -> below the NVTL threshold, the `quarter_turns`
+    best hypotheses turned by 90, 180 and 270 degrees in one more launch (a
+    scan of a street grid shows its heading only up to quarter turns) ->
+    fallback(req) when the best NVTL is still below the threshold.  `times`
+    accumulates seconds per stage.  Returns (best AlignResult,
+    used_fallback)."""
 
     def lap(key, t0):
         if times is not None:
@@ -199,6 +202,11 @@ def relative_search(matcher, net: ProposalNet, req: Request, dev: DeviceRequest,
         again = matcher.align_batch_device(ptr, n, [r.pose_vector for r in res[:polish]])
         res = sorted(again + res, key=lambda r: -r.nvtl)
         t = lap("polish", t)
+    if res[0].nvtl < nvtl_threshold and quarter_turns > 0:
+        turned = [np.array([p[0], p[1], p[2], 0.0, 0.0, wrap(p[5] + q * math.pi / 2)])
+                  for p in poses[:quarter_turns] for q in (1, 2, 3)]
+        res = sorted(matcher.align_batch_device(ptr, n, turned) + res, key=lambda r: -r.nvtl)
+        t = lap("quarter_turns", t)
     best = res[0]
     if best.nvtl < nvtl_threshold and fallback is not None:
         out = fallback(req), True
