@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "eigen6.hpp"
+
 namespace ndt {
 
 constexpr int kWave = 64;
@@ -630,52 +632,74 @@ __device__ inline void sym_eigen(const double* a_in, double* ev, double* v) {
 //  - wave 1 (one lane) factors -H - 1e-10 I and -H (Cholesky).  Success means
 //    every eigenvalue of H is below -1e-10, so the pseudo-inverse is the
 //    plain inverse and a Cholesky solve of (-H) d = g gives the same step;
-//  - wave 0 runs a parallel cyclic Jacobi on H with element (i, j) of A and
-//    of the eigenvector matrix V in lane 6 i + j.  Each round applies 3
-//    disjoint rotations (a 1-factorisation of K6, 5 rounds per sweep) with
-//    lane shuffles.  It stops early when wave 1 has found H negative
-//    definite; otherwise the step is formed in the eigenbasis with the
-//    reference's clamp.
-// The eigenvectors of the last completed decomposition are kept as the next
-// call's starting basis (A = V^T H V), which for the slowly changing
-// Hessians of successive Newton iterations is nearly diagonal.  Any
-// orthonormal V is a valid start.  A call that takes the Cholesky step keeps
-// the previous basis: its Jacobi may have stopped at a point that depends
-// on timing, and every workgroup of a slot must hold the same state.
+//  - wave 0 decomposes H by Householder tridiagonalisation and implicit QL
+//    (eigen6_ql): every lane runs the same scalar recurrence, carries
+//    y = Q^T (-g) along, and lane r < 6 accumulates row r of the eigenvector
+//    matrix Q.  The step in the eigenbasis with the reference's clamp is then
+//    d_r = sum_k Q[r][k] y_k / lambda'_k, formed by lane r without any
+//    cross-lane reduction.
+// The choice between the two depends on H alone, so every workgroup of a
+// slot takes the same one.
 struct SolveScratch {
-  double V[36];    // starting basis: eigenvectors (columns), row-major
-  double Vn[36];   // this call's eigenvectors
-  double T[36];
-  double L[36];
   double dc[6];    // Cholesky step
-  int have_basis;  // V holds the eigenvectors of the previous call's matrix
-  int sweeps;      // Jacobi sweeps run (profiling)
+  int sweeps;      // QL steps taken (profiling)
   int solves;      // calls that completed the eigendecomposition (profiling)
   int fast;        // set by wave 1 when the Cholesky step exists
   int ok;
 };
 
-// Cholesky of -H - shift*I (lower triangle of H read) into L; false if not PD.
-__device__ inline bool neg_cholesky6(const double* H, double shift, double* L) {
+// The device side of eigen6_ql's arithmetic policy.
+struct DeviceEigenMath {
+  __device__ static double rsqrt(double x) { return refine_rsqrt(x, __builtin_amdgcn_rsq(x)); }
+  __device__ static double rcp(double x) { return refine_rcp(x, __builtin_amdgcn_rcp(x)); }
+  __device__ static int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+  __device__ static double uni(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  }
+};
+
+// Cholesky of -H - shift I (lower triangle of H read) in registers and, when
+// it exists, the solve (-H - shift I) dc = g.  Returns false if -H - shift I
+// is not positive definite.
+__device__ inline bool neg_cholesky6_solve(const double* H, double shift, const double* g, double dc[6]) {
+  double L[6][6], il[6];
+#pragma unroll
   for (int j = 0; j < 6; ++j) {
     double d = -H[j * 6 + j] - shift;
-    for (int k = 0; k < j; ++k) d -= L[j * 6 + k] * L[j * 6 + k];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
     if (!(d > 0.0)) return false;
-    const double ljj = sqrt(d);
-    L[j * 6 + j] = ljj;
+    const double t = DeviceEigenMath::rsqrt(d);
+    L[j][j] = d * t;
+    il[j] = t;
+#pragma unroll
     for (int i = j + 1; i < 6; ++i) {
       double s = -H[i * 6 + j];
-      for (int k = 0; k < j; ++k) s -= L[i * 6 + k] * L[j * 6 + k];
-      L[i * 6 + j] = s / ljj;
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+      L[i][j] = s * t;
     }
+  }
+  double y[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = g[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
+    y[i] = s * il[i];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) s -= L[k][i] * dc[k];
+    dc[i] = s * il[i];
   }
   return true;
 }
-
-// Round-robin pairing of 6 indices: 5 rounds of 3 disjoint pairs (p < q).
-constexpr int kJacobiPairs[5][3][2] = {{{0, 1}, {2, 3}, {4, 5}}, {{0, 2}, {1, 4}, {3, 5}},
-                                       {{0, 3}, {1, 5}, {2, 4}}, {{0, 4}, {1, 3}, {2, 5}},
-                                       {{0, 5}, {1, 2}, {3, 4}}};
 
 // Orders LDS accesses of the lanes of one wave (no cross-wave barrier).
 __device__ inline void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup"); }
@@ -714,228 +738,54 @@ __device__ inline void make_pose_tables_wave(const double* pose, double* vals, P
   if (lane < 3) out[lane] = pose[lane];
 }
 
-// Value of x in lane l (l uniform), as a scalar.
-__device__ inline double readlane_f64(double x, int l) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Jacobi rotation (c, s) that zeroes A[p][q] of the symmetric pair block:
-// the smaller of the two angles, |phi| <= pi/4, with tan(2 phi) = 2 A_pq /
-// (A_qq - A_pp).  From r = sqrt(dd^2 + e^2) (dd = A_qq - A_pp, e = 2 A_pq):
-// cos(2 phi) = |dd| / r, c = sqrt((1 + cos(2 phi)) / 2) and
-// s = sin(2 phi) / (2 c) = sgn(dd) e / (2 c r), so two reciprocal square roots
-// and no division.
-__device__ inline void jacobi_rotation(double app, double aqq, double apq, double& c, double& s) {
-  const double dd = aqq - app, e = 2.0 * apq;
-  const double r2 = dd * dd + e * e;
-  const bool rot = apq != 0.0 && r2 > 0.0;
-  const double u = rsqrt(rot ? r2 : 1.0);
-  const double c2 = 0.5 + 0.5 * (fabs(dd) * u);
-  const double ic = rsqrt(c2);
-  c = rot ? c2 * ic : 1.0;
-  s = rot ? 0.5 * ((dd >= 0.0 ? e : -e) * u) * ic : 0.0;
-}
-
-// Per-round pairing tables packed for lookup by a runtime index x in 0..5:
-// partner (3 bits per index), pair number (2 bits) and role (1 bit: x is the
-// q of its pair).
-struct JacobiRoundTables {
-  int partner, pair, is_q;
-};
-__host__ __device__ constexpr JacobiRoundTables jacobi_tables(int r) {
-  JacobiRoundTables t{0, 0, 0};
-  for (int k = 0; k < 3; ++k) {
-    const int p = kJacobiPairs[r][k][0], q = kJacobiPairs[r][k][1];
-    t.partner |= (q << (3 * p)) | (p << (3 * q));
-    t.pair |= (k << (2 * p)) | (k << (2 * q));
-    t.is_q |= 1 << q;
-  }
-  return t;
-}
-
-// One Jacobi round R with element (i, j) of A and of V in lane 6 i + j
-// (lanes 36..63 hold zeros).  Lanes 0..2 compute the round's three rotations;
-// every lane then gathers the four entries of A and two of V that its new
-// value depends on:
-//   A'_ij = c_i c_j A_ij + c_i o_j A_i,mj + o_i c_j A_mi,j + o_i o_j A_mi,mj
-//   V'_ij = c_j V_ij + o_j V_i,mj
-// with m the partner index and o = -s for the p of a pair, +s for the q.
-// The rotated pair entries are set to zero.
-template <int R>
-__device__ inline void jacobi_round(double& aij, double& vij, int i, int j, bool live, int lane) {
-  constexpr JacobiRoundTables T = jacobi_tables(R);
-  constexpr int P0 = kJacobiPairs[R][0][0], Q0 = kJacobiPairs[R][0][1];
-  constexpr int P1 = kJacobiPairs[R][1][0], Q1 = kJacobiPairs[R][1][1];
-  constexpr int P2 = kJacobiPairs[R][2][0], Q2 = kJacobiPairs[R][2][1];
-  // rotation k in lane k
-  const int rp = lane == 0 ? P0 : (lane == 1 ? P1 : P2);
-  const int rq = lane == 0 ? Q0 : (lane == 1 ? Q1 : Q2);
-  const double app = __shfl(aij, rp * 7, kWave);
-  const double aqq = __shfl(aij, rq * 7, kWave);
-  const double apq = __shfl(aij, rp * 6 + rq, kWave);
-  double c, s;
-  jacobi_rotation(app, aqq, apq, c, s);
-  const int mi = (T.partner >> (3 * i)) & 7, mj = (T.partner >> (3 * j)) & 7;
-  const int ki = (T.pair >> (2 * i)) & 3, kj = (T.pair >> (2 * j)) & 3;
-  const double ci = __shfl(c, ki, kWave), si = __shfl(s, ki, kWave);
-  const double cj = __shfl(c, kj, kWave), sj = __shfl(s, kj, kWave);
-  const double oi = ((T.is_q >> i) & 1) ? si : -si;
-  const double oj = ((T.is_q >> j) & 1) ? sj : -sj;
-  const double a_imj = __shfl(aij, i * 6 + mj, kWave);
-  const double a_mij = __shfl(aij, mi * 6 + j, kWave);
-  const double a_mimj = __shfl(aij, mi * 6 + mj, kWave);
-  const double v_imj = __shfl(vij, i * 6 + mj, kWave);
-  const double an = ci * cj * aij + ci * oj * a_imj + oi * cj * a_mij + oi * oj * a_mimj;
-  aij = (!live || mi == j) ? 0.0 : an;
-  vij = live ? cj * vij + oj * v_imj : 0.0;
-}
-
 // All threads of the workgroup call this, after newton_step_reset and a
 // barrier; waves 0 and 1 do the work.  H (36, lower triangle read) and g (6)
 // are in LDS; on return W.ok says whether a step exists and d (LDS) holds it.
 __device__ inline void newton_step_wg(const double* H, const double* g, double* d, SolveScratch& W) {
   const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
   if (wave == 1) {
-    if (lane == 0 && neg_cholesky6(H, 1e-10, W.L) && neg_cholesky6(H, 0.0, W.L)) {
-      double y[6];
-      for (int i = 0; i < 6; ++i) {
-        double s = g[i];
-        for (int k = 0; k < i; ++k) s -= W.L[i * 6 + k] * y[k];
-        y[i] = s / W.L[i * 6 + i];
-      }
-      for (int i = 5; i >= 0; --i) {
-        double s = y[i];
-        for (int k = i + 1; k < 6; ++k) s -= W.L[k * 6 + i] * W.dc[k];
-        W.dc[i] = s / W.L[i * 6 + i];
-      }
+    // Lane 0 tests -H - 1e-10 I, lane 1 factors -H and solves, at once.
+    double dc[6];
+    const bool pd = lane < 2 && neg_cholesky6_solve(H, lane == 0 ? 1e-10 : 0.0, g, dc);
+    if ((__ballot(pd) & 3ull) == 3ull && lane == 1) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) W.dc[i] = dc[i];
       __hip_atomic_store(&W.fast, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   } else if (wave == 0) {
-    const bool live = lane < 36;
-    const int i = live ? lane / 6 : 0, j = live ? lane % 6 : 0;
-    double aij, vij;
-    if (W.have_basis) {
-      // T = Hs V, then A = V^T T (Hs: H symmetrised from its lower triangle)
-      double t = 0.0;
+    double mg[6];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) t += (i >= k ? H[i * 6 + k] : H[k * 6 + i]) * W.V[k * 6 + j];
-      if (live) W.T[lane] = t;
-      wave_lds_fence();
-      aij = 0.0;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) aij += W.V[k * 6 + i] * W.T[k * 6 + j];
-      vij = W.V[i * 6 + j];
-    } else {
-      aij = i >= j ? H[i * 6 + j] : H[j * 6 + i];
-      vij = i == j ? 1.0 : 0.0;
-    }
-    if (!live) aij = vij = 0.0;
-    bool fast_exit = false;
-    for (int sweep = 0; sweep < 60; ++sweep) {
-      double off = (i < j) ? aij * aij : 0.0, dg = (live && i == j) ? aij * aij : 0.0;
-      // The sweep's first round runs on copies while the convergence sums
-      // travel through the butterfly; it is kept only if the sweep goes on.
-      double a0 = aij, v0 = vij;
-      jacobi_round<0>(a0, v0, i, j, live, lane);
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1) {
-        off += __shfl_xor(off, o, kWave);
-        dg += __shfl_xor(dg, o, kWave);
-      }
-      off = readlane_f64(off, 0);
-      dg = readlane_f64(dg, 0);
-      // Stop once the off-diagonal norm is 1e-15 of the diagonal's: the
-      // eigenvalue error is then of order 1e-30 relative and the eigenvector
-      // error of order 1e-15 times the condition of the gaps, below what the
-      // step needs; one more sweep would only polish rounding.
-      if (off == 0.0 || off <= 1e-30 * dg) break;
-      // Nearly diagonal (off-diagonal norm <= 1e-8 of the diagonal's): the
-      // last sweep can be replaced by the first-order eigenvector update
-      // V <- V (I + E), E_kl = A_kl / (A_ll - A_kk), whose eigenvector and
-      // orthogonality errors are O(E^2) and eigenvalue errors O(A_kl E_kl);
-      // taken only when every |E_kl| <= 1e-8, i.e. errors at rounding level.
-      if (off <= 1e-16 * dg) {
-        const double dii = __shfl(aij, 7 * i, kWave), djj = __shfl(aij, 7 * j, kWave);
-        const double gap = djj - dii;
-        const double e = (live && i != j && gap != 0.0) ? aij / gap : 0.0;
-        const bool small = !live || i == j || (gap != 0.0 && fabs(e) <= 1e-8);
-        if (__ballot(small) == ~0ull) {
-          double vn = vij;
-#pragma unroll
-          for (int k = 0; k < 6; ++k) vn += __shfl(vij, i * 6 + k, kWave) * __shfl(e, k * 6 + j, kWave);
-          vij = live ? vn : 0.0;
-          break;
-        }
-      }
-      if (__hip_atomic_load(&W.fast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-        fast_exit = true;
-        break;
-      }
-      if (lane == 0) ++W.sweeps;
-      aij = a0;
-      vij = v0;
-      jacobi_round<1>(aij, vij, i, j, live, lane);
-      jacobi_round<2>(aij, vij, i, j, live, lane);
-      jacobi_round<3>(aij, vij, i, j, live, lane);
-      jacobi_round<4>(aij, vij, i, j, live, lane);
-    }
-    if (live) W.Vn[lane] = vij;
-    if (!fast_exit) {
-      // eigenvalues A_kk (lane 7k), the reference's clamp, and the step in
-      // the eigenbasis: d = sum_k (V_k . (-g)) / lam_k V_k
-      double lam[6];
+    for (int k = 0; k < 6; ++k) mg[k] = -g[k];
+    Eigen6Row o;
+    // Stops between QL blocks once wave 1 has the Cholesky step.
+    eigen6_ql<DeviceEigenMath>(H, mg, lane, o, [&W] {
+      return __hip_atomic_load(&W.fast, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+    });
+    if (o.ok >= 0) {
+      // The reference's clamp (newton.rs:119-144) and the SVD solve's
+      // tolerance: d = sum_k Q_k (Q_k . -g) / lambda'_k over |lambda'_k| > 1e-10.
       bool neg_def = true;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        lam[k] = readlane_f64(aij, 7 * k);
-        neg_def &= lam[k] < 0.0;
-      }
-      double max_sv = 0.0;
+      for (int k = 0; k < 6; ++k) neg_def &= o.lam[k] < 0.0;
+      double max_sv = 0.0, dr = 0.0;
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        lam[k] = (!neg_def && lam[k] > -1e-3) ? -1e-3 : lam[k];
-        max_sv = fmax(max_sv, fabs(lam[k]));
+        const double lk = (!neg_def && o.lam[k] > -1e-3) ? -1e-3 : o.lam[k];
+        max_sv = fmax(max_sv, fabs(lk));
+        dr += fabs(lk) > 1e-10 ? o.q[k] * (o.y[k] * DeviceEigenMath::rcp(lk)) : 0.0;
       }
-      wave_lds_fence();
-      // lane k < 6: coefficient of eigenvector k
-      double cf = 0.0;
-      if (lane < 6) {
-        double proj = 0.0;
-#pragma unroll
-        for (int a = 0; a < 6; ++a) proj += W.Vn[a * 6 + lane] * (-g[a]);
-        double lk = lam[0];
-#pragma unroll
-        for (int k = 1; k < 6; ++k)
-          if (lane == k) lk = lam[k];
-        cf = fabs(lk) > 1e-10 ? proj / lk : 0.0;
-      }
-      if (lane < 6) W.T[lane] = cf;
-      wave_lds_fence();
       if (max_sv >= 1e-10) {
-        if (lane < 6) {
-          double di = 0.0;
-#pragma unroll
-          for (int k = 0; k < 6; ++k) di += W.Vn[lane * 6 + k] * W.T[k];
-          d[lane] = di;
-        }
+        if (lane < 6) d[lane] = dr;
         if (lane == 0) W.ok = 1;
       }
       if (lane == 0) ++W.solves;
     }
+    if (lane == 0) W.sweeps += o.iters;
   }
   __syncthreads();
-  // W.fast depends on H alone, so every workgroup of a slot makes the same
-  // choice: the Cholesky step (previous basis kept) or the eigenbasis step
-  // (its eigenvectors become the next start).
   if (W.fast) {
     if (threadIdx.x < 6) d[threadIdx.x] = W.dc[threadIdx.x];
     if (threadIdx.x == 0) W.ok = 1;
-  } else {
-    if (threadIdx.x < 36) W.V[threadIdx.x] = W.Vn[threadIdx.x];
-    if (threadIdx.x == 0) W.have_basis = 1;
   }
   __syncthreads();
 }

@@ -274,55 +274,65 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
 #pragma unroll
     for (int k = 0; k < 9; ++k) M[k] = 0;
     uint32_t np = 0;
-    auto pair = [&](int32_t s, VoxelRecord rec) {
+    // One pair's terms: sraw = e_raw, and the a and M increments in the
+    // common form a += w u, M += w C^T (point-to-distribution only) +
+    // q u u^T.  ok is false when the pair is counted but contributes zero.
+    auto eval = [&](int32_t s, const VoxelRecord& rec, R& sraw, R& w, R& q, R u[3], bool& ok) {
       // residual in double (the point is placed in double), then R
       const R x0 = (R)(t[0] - (double)rec.m0.x), x1 = (R)(t[1] - (double)rec.m0.y), x2 = (R)(t[2] - (double)rec.m0.z);
-      ++corr;
-      ++np;
       if (cfg.metric == 1) {
-        const R n0 = g.axis[3 * s], n1 = g.axis[3 * s + 1], n2 = g.axis[3 * s + 2];
-        const R d = x0 * n0 + x1 * n1 + x2 * n2;
+        u[0] = g.axis[3 * s];
+        u[1] = g.axis[3 * s + 1];
+        u[2] = g.axis[3 * s + 2];
+        const R d = x0 * u[0] + x1 * u[1] + x2 * u[2];
         const R dsq = d * d;
         const R e_raw = exp_r(-d2 * dsq * (R)0.5);
         const R nz = d2 * e_raw;
-        if (!(nz >= (R)0 && nz <= (R)1) || !isfinite(nz)) return;  // counted, contributes zero
+        ok = (nz >= (R)0 && nz <= (R)1) && isfinite(nz);
         const R e = d1 * d2 * e_raw;
-        sc += -d1 * e_raw;
-        a0 += e * d * n0;
-        a1 += e * d * n1;
-        a2 += e * d * n2;
-        if (kHess) {
-          const R f = e * (-d2 * dsq + (R)1);
-          const R nn[3] = {n0, n1, n2};
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) M[3 * r + c] += f * (nn[r] * nn[c]);
-        }
+        sraw = e_raw;
+        w = e * d;
+        q = e * (-d2 * dsq + (R)1);
       } else {
         const float4 m0 = rec.m0, m1 = rec.m1, m2 = rec.m2;
         const R C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
-        const R xc0 = x0 * C[0] + x1 * C[3] + x2 * C[6];
-        const R xc1 = x0 * C[1] + x1 * C[4] + x2 * C[7];
-        const R xc2 = x0 * C[2] + x1 * C[5] + x2 * C[8];
-        const R xcx = xc0 * x0 + xc1 * x1 + xc2 * x2;
+        u[0] = x0 * C[0] + x1 * C[3] + x2 * C[6];
+        u[1] = x0 * C[1] + x1 * C[4] + x2 * C[7];
+        u[2] = x0 * C[2] + x1 * C[5] + x2 * C[8];
+        const R xcx = u[0] * x0 + u[1] * x1 + u[2] * x2;
         const R e_raw = exp_r(-d2 * xcx * (R)0.5);
         const R nz = d2 * e_raw;
-        if (!(nz >= (R)0 && nz <= (R)1) || !isfinite(nz)) return;  // counted, contributes zero
+        ok = (nz >= (R)0 && nz <= (R)1) && isfinite(nz);
         const R e = nz * d1;
-        sc += -d1 * e_raw;
-        a0 += e * xc0;
-        a1 += e * xc1;
-        a2 += e * xc2;
-        if (kHess) {
-          const R xc[3] = {xc0, xc1, xc2};
-          const R ed = -d2 * e;
+        sraw = e_raw;
+        w = e;
+        q = -d2 * e;
+      }
+    };
+    auto accum = [&](const VoxelRecord& rec, R sraw, R w, R q, const R u[3]) {
+      sc += -d1 * sraw;
+      a0 += w * u[0];
+      a1 += w * u[1];
+      a2 += w * u[2];
+      if (kHess) {
+        if (cfg.metric == 1) {
 #pragma unroll
           for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) M[3 * r + c] += e * C[3 * c + r] + ed * (xc[r] * xc[c]);
+            for (int c = 0; c < 3; ++c) M[3 * r + c] += q * (u[r] * u[c]);
+        } else {
+          const float4 m0 = rec.m0, m1 = rec.m1, m2 = rec.m2;
+          const R C[9] = {m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) M[3 * r + c] += w * C[3 * c + r] + q * (u[r] * u[c]);
         }
       }
+    };
+    auto staged_rec = [&](uint32_t v) {
+      return VoxelRecord{st.rec[(3 * v) * kBlock + tid], st.rec[(3 * v + 1) * kBlock + tid],
+                         st.rec[(3 * v + 2) * kBlock + tid]};
     };
     // Neighbours kStageK at a time, in enumeration order; a point with more
     // (rare: at most 27 exist) enumerates again for the next ones.
@@ -339,10 +349,27 @@ __device__ inline void derivative_points(const GridView& g, const NewtonConfig& 
         }
         ++seen;
       });
-      for (uint32_t v = 0; v < staged; ++v) {
-        const VoxelRecord r{st.rec[(3 * v) * kBlock + tid], st.rec[(3 * v + 1) * kBlock + tid],
-                            st.rec[(3 * v + 2) * kBlock + tid]};
-        pair(st.slot[v * kBlock + tid], r);
+      corr += staged;
+      np += staged;
+      // Two pairs per step: their terms are independent dependency chains
+      // (the exponential above all), so evaluating both before accumulating
+      // either overlaps them; the sums still take the pairs in order.
+      uint32_t v = 0;
+      for (; v + 1 < staged; v += 2) {
+        const VoxelRecord r0 = staged_rec(v), r1 = staged_rec(v + 1);
+        R s0, w0, q0, u0[3], s1, w1, q1, u1[3];
+        bool ok0, ok1;
+        eval(st.slot[v * kBlock + tid], r0, s0, w0, q0, u0, ok0);
+        eval(st.slot[(v + 1) * kBlock + tid], r1, s1, w1, q1, u1, ok1);
+        if (ok0) accum(r0, s0, w0, q0, u0);
+        if (ok1) accum(r1, s1, w1, q1, u1);
+      }
+      if (v < staged) {
+        const VoxelRecord r0 = staged_rec(v);
+        R s0, w0, q0, u0[3];
+        bool ok0;
+        eval(st.slot[v * kBlock + tid], r0, s0, w0, q0, u0, ok0);
+        if (ok0) accum(r0, s0, w0, q0, u0);
       }
       if (seen <= skip + kStageK) break;
     }
@@ -758,7 +785,6 @@ __device__ __attribute__((always_inline)) void align_slot(const GridView& g, con
     S.status = -1;
     S.last_corr = 0;
     S.corr_total = 0;
-    s_solve.have_basis = 0;
     s_solve.sweeps = 0;
     s_solve.solves = 0;
     newton_step_reset(s_solve);
