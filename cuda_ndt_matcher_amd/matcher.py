@@ -261,11 +261,12 @@ class NdtScanMatcher:
             self._lib.ndt_hip_destroy(self._h)
             self._h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # At interpreter shutdown the HIP runtime (and a profiler's hooks) may
         # already be torn down: the handle is left to the process exit then.
         # Callers release handles with close() (or a with block) before that.
-        if sys.is_finalizing():
+        # (sys itself may be cleared by then, hence the bound default.)
+        if _finalizing():
             return
         try:
             self.close()

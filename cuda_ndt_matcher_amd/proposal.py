@@ -49,9 +49,11 @@ class ProposalNet:
             self._lib.ndt_hip_mlp_destroy(h)
             self._h = None
 
-    def __del__(self):
-        # No HIP calls during interpreter shutdown (the runtime may be gone).
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # No HIP calls during interpreter shutdown (the runtime may be gone;
+        # module globals such as sys may already be cleared, hence the bound
+        # default argument).
+        if _finalizing():
             return
         self.close()
 
