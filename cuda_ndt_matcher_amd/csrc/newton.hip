@@ -670,7 +670,7 @@ __device__ inline double reg_hessian(const RegTerms& r, int i, int j) {
   return (i > 1 || j > 1) ? 0.0 : (i == 0 && j == 0 ? r.h00 : (i == 1 && j == 1 ? r.h11 : r.h01));
 }
 
-__device__ void regularization_terms(const NewtonConfig& cfg, const double* pose, uint32_t corr, double* rs,
+__device__ __attribute__((always_inline)) inline void regularization_terms(const NewtonConfig& cfg, const double* pose, uint32_t corr, double* rs,
                                      double rg[6], double rH[36]) {
   const RegTerms r = regularization_xy(cfg, pose, corr);
   *rs = r.s;
