@@ -927,8 +927,8 @@ def main():
               f"{err_r:.3f} deg, {ok} particles within 2 cm", file=sys.stderr, flush=True)
         if not args.no_cpu:
             extra["particle_search"]["cpu_baseline"] = cpu_particle_baseline(wl.map_points, [(wl.scan, parts)])
-            print(f"[bench] init pose CPU: {extra['init_pose']['cpu_baseline']['value']:.3f} solves/s on "
-                  f"{extra['init_pose']['cpu_baseline']['cores']} threads", file=sys.stderr, flush=True)
+            print(f"[bench] particle search CPU: {extra['particle_search']['cpu_baseline']['value']:.3f} solves/s on "
+                  f"{extra['particle_search']['cpu_baseline']['cores']} threads", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,

@@ -182,14 +182,25 @@ __device__ inline uint32_t box_index(const GridView& g, int32_t x, int32_t y, in
   return (uint32_t)(x - g.borg[0]) + g.bdim[0] * ((uint32_t)(y - g.borg[1]) + g.bdim[1] * (uint32_t)(z - g.borg[2]));
 }
 
+// Shape of one cell-table load stage: E candidate entries in flight, then
+// the records of the accepted ones in groups of R.  The Newton kernel runs one
+// wave per SIMD whatever the shape, so it takes the wide one (fewer dependent
+// rounds); the scoring kernels keep two waves per SIMD with the narrow one.
+template <int E, int R>
+struct ProbeShape {
+  static constexpr int kEnt = E, kRec = R;
+};
+using ProbeWide = ProbeShape<12, 6>;
+using ProbeNarrow = ProbeShape<8, 8>;
+
 // Cell-table visit of the candidate cells around the cell with box index
-// `lin`, sixteen at a time: their 16-byte entries (mean, slot) are loaded
-// together, then the records of the accepted ones, eight at a time (a
-// second group only when a lane accepted a candidate past its eighth).
+// `lin`, P::kEnt at a time: their 16-byte entries (mean, slot) are loaded
+// together, then the records of the accepted ones, P::kRec at a time (a
+// second group only when a lane accepted a candidate past its first kRec).
 // Candidates are visited in increasing cell order, as in probe_cells.
-template <class A, class F>
+template <class P = ProbeWide, class A, class F>
 __device__ inline void probe_dense(const GridView& g, uint32_t lin, uint32_t cand, A&& accept, F&& visit) {
-  constexpr int kEnt = 12, kRec = 6;
+  constexpr int kEnt = P::kEnt, kRec = P::kRec;
   const int32_t sy = (int32_t)g.bdim[0], sz = (int32_t)(g.bdim[0] * g.bdim[1]);
 #pragma unroll 1
   while (cand) {
@@ -208,28 +219,44 @@ __device__ inline void probe_dense(const GridView& g, uint32_t lin, uint32_t can
     }
 #pragma unroll
     for (int b = 0; b < kEnt; ++b) ok[b] = ok[b] && accept(ent[b].x, ent[b].y, ent[b].z);
-    uint32_t slot[kEnt];
+    if constexpr (kRec >= kEnt) {
+      // One group: every accepted candidate's record in flight at once.
+      float4 m0[kEnt], m1[kEnt], m2[kEnt];
 #pragma unroll
-    for (int b = 0; b < kEnt; ++b) slot[b] = __float_as_uint(ent[b].w);
-    auto records = [&](auto h) {
-      constexpr int o = decltype(h)::value * kRec;
-      float4 m0[kRec], m1[kRec], m2[kRec];
-#pragma unroll
-      for (int b = 0; b < kRec; ++b)
-        if (ok[o + b]) {
-          m0[b] = g.vox[3 * slot[o + b]];
-          m1[b] = g.vox[3 * slot[o + b] + 1];
-          m2[b] = g.vox[3 * slot[o + b] + 2];
+      for (int b = 0; b < kEnt; ++b)
+        if (ok[b]) {
+          const uint32_t sl = __float_as_uint(ent[b].w);
+          m0[b] = g.vox[3 * sl];
+          m1[b] = g.vox[3 * sl + 1];
+          m2[b] = g.vox[3 * sl + 2];
         }
 #pragma unroll
-      for (int b = 0; b < kRec; ++b)
-        if (ok[o + b]) visit((int32_t)slot[o + b], VoxelRecord{m0[b], m1[b], m2[b]});
-    };
-    records(std::integral_constant<int, 0>{});
-    bool late = false;
+      for (int b = 0; b < kEnt; ++b)
+        if (ok[b]) visit((int32_t)__float_as_uint(ent[b].w), VoxelRecord{m0[b], m1[b], m2[b]});
+    } else {
+      uint32_t slot[kEnt];
 #pragma unroll
-    for (int b = kRec; b < kEnt; ++b) late |= ok[b];
-    if (__any(late)) records(std::integral_constant<int, 1>{});
+      for (int b = 0; b < kEnt; ++b) slot[b] = __float_as_uint(ent[b].w);
+      auto records = [&](auto h) {
+        constexpr int o = decltype(h)::value * kRec;
+        float4 m0[kRec], m1[kRec], m2[kRec];
+#pragma unroll
+        for (int b = 0; b < kRec; ++b)
+          if (ok[o + b]) {
+            m0[b] = g.vox[3 * slot[o + b]];
+            m1[b] = g.vox[3 * slot[o + b] + 1];
+            m2[b] = g.vox[3 * slot[o + b] + 2];
+          }
+#pragma unroll
+        for (int b = 0; b < kRec; ++b)
+          if (ok[o + b]) visit((int32_t)slot[o + b], VoxelRecord{m0[b], m1[b], m2[b]});
+      };
+      records(std::integral_constant<int, 0>{});
+      bool late = false;
+#pragma unroll
+      for (int b = kRec; b < kEnt; ++b) late |= ok[b];
+      if (__any(late)) records(std::integral_constant<int, 1>{});
+    }
   }
 }
 
@@ -237,7 +264,7 @@ __device__ inline void probe_dense(const GridView& g, uint32_t lin, uint32_t can
 // through the cell tables when they are built and otherwise through the
 // bitmap and the hash.  Nothing is visited when no occupied cell lies within
 // one cell of k.
-template <class A, class F>
+template <class P = ProbeWide, class A, class F>
 __device__ inline void probe_block(const GridView& g, int32_t kx, int32_t ky, int32_t kz, uint32_t mask, A&& accept,
                                    F&& visit) {
   if (kx < g.cmin[0] - 1 || kx > g.cmax[0] + 1 || ky < g.cmin[1] - 1 || ky > g.cmax[1] + 1 ||
@@ -245,7 +272,7 @@ __device__ inline void probe_block(const GridView& g, int32_t kx, int32_t ky, in
     return;
   if (g.cell_ent) {
     const uint32_t lin = box_index(g, kx, ky, kz);
-    probe_dense(g, lin, mask & g.cell_nbr[lin], accept, visit);
+    probe_dense<P>(g, lin, mask & g.cell_nbr[lin], accept, visit);
     return;
   }
   if (g.bitmap) mask &= occupancy27(g, kx, ky, kz);
@@ -291,7 +318,7 @@ __device__ inline float dist2_f32(float qx, float qy, float qz, float mx, float 
 // res of q belongs to a cell whose index differs from floor(q/res) by at most
 // one on each axis; floor is taken of the f64 quotient, which is exact for
 // f32 inputs.  Cells whose box is farther than res from q are skipped.
-template <class F>
+template <class P = ProbeWide, class F>
 __device__ inline void for_each_radius(const GridView& g, float qx, float qy, float qz, F&& visit) {
   const double res = (double)g.res;
   const int32_t kx = sat_i32d(floor((double)qx / res));
@@ -320,7 +347,7 @@ __device__ inline void for_each_radius(const GridView& g, float qx, float qy, fl
         if (sxy + sz[c] <= lim) mask |= 1u << (9 * a + 3 * b + c);
     }
   const float r2 = __fmul_rn(g.res, g.res);
-  probe_block(
+  probe_block<P>(
       g, kx, ky, kz, mask, [&](float mx, float my, float mz) { return dist2_f32(qx, qy, qz, mx, my, mz) < r2; },
       visit);
 }
@@ -336,27 +363,27 @@ __host__ __device__ constexpr uint32_t direct_mask(int mode) {
                                 : (((1u << 27) - 1) & ~(1u << 13)));
 }
 
-template <class F>
+template <class P = ProbeWide, class F>
 __device__ inline void for_each_direct(const GridView& g, int mode, float qx, float qy, float qz, F&& visit) {
   const int32_t kx = voxel_key(qx, g.res), ky = voxel_key(qy, g.res), kz = voxel_key(qz, g.res);
-  probe_block(g, kx, ky, kz, direct_mask(mode), [](float, float, float) { return true; }, visit);
+  probe_block<P>(g, kx, ky, kz, direct_mask(mode), [](float, float, float) { return true; }, visit);
 }
 
-template <class F>
+template <class P = ProbeWide, class F>
 __device__ inline void for_each_neighbor(const GridView& g, int mode, float qx, float qy, float qz, F&& visit) {
   if (mode == 0)
-    for_each_radius(g, qx, qy, qz, visit);
+    for_each_radius<P>(g, qx, qy, qz, visit);
   else
-    for_each_direct(g, mode, qx, qy, qz, visit);
+    for_each_direct<P>(g, mode, qx, qy, qz, visit);
 }
 
 // compute_max_neighbor_score's cell set (scoring/nvtl.rs:160-190): 3x3x3
 // cells around floor(x * (1/res)) in f64, no distance test.
-template <class F>
+template <class P = ProbeWide, class F>
 __device__ inline void for_each_cell27(const GridView& g, double x, double y, double z, F&& visit) {
   const double inv = 1.0 / (double)g.res;
   const int32_t vx = sat_i32d(floor(x * inv)), vy = sat_i32d(floor(y * inv)), vz = sat_i32d(floor(z * inv));
-  probe_block(g, vx, vy, vz, (1u << 27) - 1, [](float, float, float) { return true; }, visit);
+  probe_block<P>(g, vx, vy, vz, (1u << 27) - 1, [](float, float, float) { return true; }, visit);
 }
 
 // ---------------------------------------------------------------------------

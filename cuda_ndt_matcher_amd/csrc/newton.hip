@@ -1221,9 +1221,9 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
         any = true;
       };
       if (kind == kScoreNvtl)
-        for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], vis);
+        for_each_radius<ProbeNarrow>(g, (float)x[0], (float)x[1], (float)x[2], vis);
       else
-        for_each_cell27(g, x[0], x[1], x[2], vis);
+        for_each_cell27<ProbeNarrow>(g, x[0], x[1], x[2], vis);
       if (any) {
         sum += best;
         cnt += 1.0;
@@ -1231,7 +1231,7 @@ __global__ __launch_bounds__(kBlock) void score_kernel(GridView g, double d1, do
     } else {
       double ps = 0.0;
       double pc = 0.0;
-      for_each_radius(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, VoxelRecord r) {
+      for_each_radius<ProbeNarrow>(g, (float)x[0], (float)x[1], (float)x[2], [&](int32_t, VoxelRecord r) {
         ps += point_score(x, r.m0, r.m1, r.m2, d1, d2);
         pc += 1.0;
       });
@@ -1268,9 +1268,9 @@ __global__ __launch_bounds__(kBlock) void lookup_kernel(GridView g, int mode, co
     ++c;
   };
   if (mode == 4)
-    for_each_cell27(g, (double)q[3 * i], (double)q[3 * i + 1], (double)q[3 * i + 2], vis);
+    for_each_cell27<ProbeNarrow>(g, (double)q[3 * i], (double)q[3 * i + 1], (double)q[3 * i + 2], vis);
   else
-    for_each_neighbor(g, mode, q[3 * i], q[3 * i + 1], q[3 * i + 2], vis);
+    for_each_neighbor<ProbeNarrow>(g, mode, q[3 * i], q[3 * i + 1], q[3 * i + 2], vis);
   for (uint32_t k = c; k < max_k; ++k) idx[(size_t)i * max_k + k] = -1;
   counts[i] = c;
 }

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from oracle import parity
 from tests.test_ingest_host import KAT_CASES
 
 pytestmark = pytest.mark.gpu
@@ -123,23 +124,30 @@ def test_filter_device_feeds_align():
     assert np.array_equal(c.pose_vector, b.pose_vector)
 
 
-def test_dual_matcher_rebuilds_while_aligning():
+def test_dual_matcher_rebuilds_while_aligning(full_wl, full_oracle, oracle_mod):
     """A 5M-point map is built on the standby handle while the active one
-    keeps aligning on the 200k-point map: those alignments equal a separate
-    matcher's on the old map, and after the swap they equal a fresh matcher's
-    on the new map."""
+    keeps aligning on the 200k-point map.  Those alignments equal a separate
+    matcher's on the old map bit for bit, and that result meets the oracle on
+    the old map.  After the swap, alignments from starts the oracle converges
+    from on the new map (the true pose plus N(0, 5 cm) on x, y, z) move from
+    their start, meet the oracle on the new map and equal a fresh matcher's."""
     from cuda_ndt_matcher_amd import NdtScanMatcher, NdtScanMatcherConfig, synth
     from cuda_ndt_matcher_amd.dual import DualMatcher
 
     small = synth.make_workload("small", 0)
-    big = synth.make_workload("full", 0)
+    big = full_wl
     cfg = NdtScanMatcherConfig(resolution=2.0, max_iterations=30, trans_epsilon=0.01)
+    p = oracle_mod.params()
     _case("dual matcher: background 5M-point build during alignments")
     ref_old = NdtScanMatcher(cfg)
     ref_old.set_target(small.map_points)
     # align_batch does not swap (dual_ndt_manager.rs:314-321), so every call in
     # the loop runs on the old map
     want_old = ref_old.align_batch(small.scan, [small.guess])[0]
+    ref_old.close()
+    o_old = oracle_mod.Grid(small.map_points, 2.0).align(small.scan, small.guess, p)
+    why = parity.mismatch(want_old, o_old)
+    assert why is None, f"old map: {why}"
     d = DualMatcher(cfg)
     d.set_target(small.map_points)
     assert d.start_background_update(big.map_points)
@@ -156,7 +164,17 @@ def test_dual_matcher_rebuilds_while_aligning():
     assert st.swap_count == 1 and not st.in_progress
     ref_new = NdtScanMatcher(cfg)
     ref_new.set_target(big.map_points)
-    want_new = ref_new.align(big.scan, big.guess)
-    got = d.align(big.scan, big.guess)
-    assert np.array_equal(got.pose_vector, want_new.pose_vector) and got.iterations == want_new.iterations
+    for k in range(3):
+        rng = np.random.default_rng(400 + k)
+        start = np.asarray(big.true_pose, np.float64) + np.concatenate([rng.normal(0, 0.05, 3), np.zeros(3)])
+        _case(f"dual matcher: after the swap, converging start {k}")
+        got = d.align(big.scan, start)
+        o = full_oracle.align(big.scan, start, p)
+        assert o.status == 0, f"start {k} does not converge in the oracle"
+        why = parity.mismatch(got, o)
+        assert why is None, f"new map, start {k}: {why}"
+        assert np.abs(np.asarray(got.pose_vector[:3]) - start[:3]).max() > 1e-3, "the alignment did not move"
+        want_new = ref_new.align(big.scan, start)
+        assert np.array_equal(got.pose_vector, want_new.pose_vector) and got.iterations == want_new.iterations
+    ref_new.close()
     d.close()
