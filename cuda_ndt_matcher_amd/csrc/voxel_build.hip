@@ -14,7 +14,8 @@
 //   4. segments       : head flags, tile counts, scan, segment starts.
 //   5. voxel_stats    : one thread per segment: shifted f64 moments, mean,
 //                       covariance, 3x3 Jacobi eigen-regularisation, inverse.
-//   6. hash_insert    : open-addressing insert of every valid voxel.
+//   6. cell tables (occupancy word and mean/slot entry per cell of the box),
+//      or, when the box is too large for them, an open-addressing hash.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -899,14 +900,6 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
                      reinterpret_cast<float4*>(G.vox.ptr), reinterpret_cast<float*>(G.cov.ptr),
                      reinterpret_cast<float*>(G.axis.ptr), reinterpret_cast<uint32_t*>(G.count.ptr),
                      reinterpret_cast<int32_t*>(G.coord.ptr), reinterpret_cast<uint32_t*>(G.valid.ptr));
-  const uint32_t cap = std::max<uint32_t>(1024, next_pow2(2ull * segs));
-  if ((e = G.table.reserve((size_t)cap * sizeof(HashEntry))) != hipSuccess) return fail(e, "alloc hash");
-  hipLaunchKernelGGL(fill_u32, dim3(grid_for((size_t)cap * 8, 256, 4096)), dim3(256), 0, st,
-                     reinterpret_cast<uint32_t*>(G.table.ptr), (size_t)cap * 8, 0xFFFFFFFFu);
-  hipLaunchKernelGGL(hash_insert, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
-                     reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                     reinterpret_cast<const float4*>(G.vox.ptr), segs, reinterpret_cast<HashEntry*>(G.table.ptr),
-                     cap - 1);
   const int vb_blocks = grid_for(segs, kBuildBlock, kMaxBoundBlocks);
   hipLaunchKernelGGL(valid_bounds, dim3(vb_blocks), dim3(kBuildBlock), 0, st,
                      reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
@@ -920,8 +913,8 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   GridView& v = G.view;
   v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
   v.axis = reinterpret_cast<const float*>(G.axis.ptr);
-  v.table = reinterpret_cast<const HashEntry*>(G.table.ptr);
-  v.table_mask = cap - 1;
+  v.table = nullptr;
+  v.table_mask = 0;
   for (int a = 0; a < 3; ++a) {
     v.cmin[a] = vb[a];
     v.cmax[a] = vb[3 + a];
@@ -964,6 +957,20 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
         v.cell_ent = reinterpret_cast<const float4*>(G.cell_ent.ptr);
       }
     }
+  }
+  // The open-addressing hash serves the lookups only when there are no cell
+  // tables (box too large, or cell tables switched off).
+  if (!v.cell_ent) {
+    const uint32_t cap = std::max<uint32_t>(1024, next_pow2(2ull * segs));
+    if ((e = G.table.reserve((size_t)cap * sizeof(HashEntry))) != hipSuccess) return fail(e, "alloc hash");
+    hipLaunchKernelGGL(fill_u32, dim3(grid_for((size_t)cap * 8, 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<uint32_t*>(G.table.ptr), (size_t)cap * 8, 0xFFFFFFFFu);
+    hipLaunchKernelGGL(hash_insert, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+                       reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
+                       reinterpret_cast<const float4*>(G.vox.ptr), segs, reinterpret_cast<HashEntry*>(G.table.ptr),
+                       cap - 1);
+    v.table = reinterpret_cast<const HashEntry*>(G.table.ptr);
+    v.table_mask = cap - 1;
   }
   (void)hipEventRecord(G.ev_build1, st);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
