@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "morton.hpp"
 #include "ndt_device.hpp"
 #include "ndt_internal.hpp"
 
@@ -114,20 +115,6 @@ __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restric
   block_minmax_store(mn, mx, 0u, part);
 }
 
-// Interleaves the bits of (rx, ry, rz) with widths (bx, by, bz): bit i of each
-// axis in x, y, z order, skipping axes that have run out of bits.
-__device__ inline uint64_t compact_morton(uint32_t rx, uint32_t ry, uint32_t rz, int bx, int by, int bz) {
-  uint64_t out = 0;
-  int pos = 0;
-  const int nb = max(bx, max(by, bz));
-  for (int i = 0; i < nb; ++i) {
-    if (i < bx) out |= (uint64_t)((rx >> i) & 1u) << pos++;
-    if (i < by) out |= (uint64_t)((ry >> i) & 1u) << pos++;
-    if (i < bz) out |= (uint64_t)((rz >> i) & 1u) << pos++;
-  }
-  return out;
-}
-
 struct KeyParams {
   float res;
   float inv;  // 1 / res in f32 (mul keys)
@@ -135,6 +122,7 @@ struct KeyParams {
   int32_t kmin[3];
   int bits[3];
   int passes;
+  MortonLayout layout;  // from bits (morton_layout)
 };
 
 // voxel_downsample_cpu's cell (filtering/cpu.rs:79-81): floor of the f32
@@ -154,7 +142,7 @@ __global__ __launch_bounds__(kBuildBlock) void keygen_hist(const float* __restri
     const uint32_t rx = (uint32_t)(key_of(x, kp) - kp.kmin[0]);
     const uint32_t ry = (uint32_t)(key_of(y, kp) - kp.kmin[1]);
     const uint32_t rz = (uint32_t)(key_of(z, kp) - kp.kmin[2]);
-    const uint64_t m = compact_morton(rx, ry, rz, kp.bits[0], kp.bits[1], kp.bits[2]);
+    const uint64_t m = spread_morton(rx, ry, rz, kp.layout);
     keys[i] = (K)m;
     for (int p = 0; p < kp.passes; ++p) atomicAdd(&s_hist[p * kRadix + ((m >> (8 * p)) & 0xFF)], 1u);
   };
@@ -863,6 +851,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
     total_bits += bits;
   }
   kp.passes = std::max(1, (total_bits + 7) / 8);
+  kp.layout = morton_layout(kp.bits[0], kp.bits[1], kp.bits[2]);
   G.key_bits = total_bits;
   G.key64 = total_bits > 32;
   std::memcpy(G.key_min, kp.kmin, sizeof(kp.kmin));
@@ -1096,6 +1085,7 @@ int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distanc
     total_bits += bits;
   }
   kp.passes = std::max(1, (total_bits + 7) / 8);
+  kp.layout = morton_layout(kp.bits[0], kp.bits[1], kp.bits[2]);
   const bool key64 = total_bits > 32;
   const size_t ksz = key64 ? 8 : 4;
   if ((e = G.sort_keys_a.reserve((size_t)kept * ksz)) != hipSuccess) return fail(e, "alloc keys");
