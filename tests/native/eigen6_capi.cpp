@@ -20,7 +20,7 @@ struct HostMath {
 
 // The device's form: an estimate good to about 2^-22 (here the exact value
 // with its low 30 mantissa bits cleared, a relative error up to 2^-22) and
-// the two Newton steps of refine_rsqrt / refine_rcp.
+// the third-order step of refine_rsqrt / refine_rcp.
 double truncate22(double v) {
   uint64_t b;
   std::memcpy(&b, &v, 8);
