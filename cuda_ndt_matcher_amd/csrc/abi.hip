@@ -508,14 +508,14 @@ void ndt_hip_destroy(ndt_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   ndt::DeviceGrid& G = h->grid;
   for (DevBuf* b : {&G.owned_points, &G.sort_keys_a, &G.sort_keys_b, &G.sort_vals_a, &G.sort_vals_b, &G.hist,
-                    &G.status, &G.tile_heads, &G.seg_start, &G.scalars, &G.block_part, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
+                    &G.tile_hist, &G.tile_heads, &G.seg_start, &G.scalars, &G.block_part, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
                     &G.valid, &G.table, &G.bitmap, &G.cell_nbr, &G.cell_ent, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
                     &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
   h->gcache.reset();
   ndt::DeviceGrid& F = h->filt;
   for (DevBuf* b : {&F.owned_points, &F.sort_keys_a, &F.sort_keys_b, &F.sort_vals_a, &F.sort_vals_b, &F.hist,
-                    &F.status, &F.tile_heads, &F.seg_start, &F.scalars, &F.block_part, &h->filt_in, &h->filt_out})
+                    &F.tile_hist, &F.tile_heads, &F.seg_start, &F.scalars, &F.block_part, &h->filt_in, &h->filt_out})
     b->release();
   for (ndt::DeviceGrid* g : {&G, &F}) {
     if (g->host_scalars) (void)hipHostFree(g->host_scalars);

@@ -43,7 +43,7 @@ struct DeviceGrid {
   DevBuf owned_points;
   size_t n_points = 0;
   // sort scratch
-  DevBuf sort_keys_a, sort_keys_b, sort_vals_a, sort_vals_b, hist, status, tile_heads, seg_start, scalars;
+  DevBuf sort_keys_a, sort_keys_b, sort_vals_a, sort_vals_b, hist, tile_hist, tile_heads, seg_start, scalars;
   void* sorted_keys = nullptr;
   uint32_t* sorted_vals = nullptr;
   int key_bits = 0;

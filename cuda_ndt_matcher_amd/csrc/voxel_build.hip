@@ -8,9 +8,10 @@
 //   2. keygen_hist    : compact Morton code of the key relative to the box
 //                       (only the significant bits of each axis), written as
 //                       u32 (<= 32 bits) or u64; per-pass digit histograms.
-//   3. onesweep_pass  : stable LSD radix sort, one kernel per 8-bit digit:
-//                       wave64 ballot ranking, LDS staging, decoupled
-//                       look-back across tiles (dynamic tile ids).
+//   3. radix sort     : stable LSD, 8-bit digits; per pass sort_count (digit
+//                       counts per 4096-key tile), sort_tile_scan (prefix
+//                       over tiles), sort_scatter (wave64 ballot ranking,
+//                       LDS staging, writes in runs).
 //   4. segments       : head flags, tile counts, scan, segment starts.
 //   5. voxel_stats    : one thread per segment: shifted f64 moments, mean,
 //                       covariance, 3x3 Jacobi eigen-regularisation, inverse.
@@ -34,9 +35,7 @@ constexpr int kBuildBlock = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kBuildBlock * kSortItems;  // 4096 keys per tile
 constexpr int kRadix = 256;
-constexpr uint32_t kFlagAgg = 1u << 30;
-constexpr uint32_t kFlagInc = 2u << 30;
-constexpr uint32_t kValueMask = (1u << 30) - 1;
+static_assert(kRadix == kBuildBlock, "the sort passes give each thread one digit");
 
 __global__ void fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -181,58 +180,144 @@ __global__ __launch_bounds__(kRadix) void scan_hist(uint32_t* hist, int passes) 
   }
 }
 
-// One stable LSD pass over the 8-bit digit at `shift`.
+// One stable LSD pass over the 8-bit digit at `shift` takes three kernels:
+// sort_count (digit counts per tile of 4096 keys), sort_tile_scan (for each
+// digit, the exclusive prefix over tiles plus the digit's global offset) and
+// sort_scatter (ranks within the tile, scatter to the prefix).  No workgroup
+// waits on another: tile ids are block ids and the only cross-tile data is
+// the scanned table, written by the kernel before.
 //
 // Keys are loaded wave-striped: item i of lane l in wave w is element
 // tile*4096 + w*1024 + i*64 + l, so (wave, item, lane) order is input order
-// and the ranks below are stable.  Within a wave the rank of an item is the
-// wave's running count of its digit plus the number of lower lanes holding
-// the same digit in this round (found with 8 ballots).  Tile totals per digit
-// are published with a two-bit flag; each digit's thread looks back over
-// earlier tiles until it meets an inclusive prefix.  Tile ids come from an
-// atomic counter, so every tile a workgroup waits on belongs to a workgroup
-// that started earlier and is running.  Keys are staged in LDS in digit order
-// and then written out in runs.
+// and the ranks are stable.  Within a wave the rank of an item is the wave's
+// running count of its digit plus the number of lower lanes holding the same
+// digit in this round (found with 8 ballots).  Every load is issued before the
+// ranking starts (the ranking's LDS traffic would otherwise keep the compiler
+// from hoisting them, one memory round trip per item); indices past n are
+// clamped to n - 1 (n >= 1 whenever a tile exists) and the items marked
+// invalid.
+
+// Lanes of this wave whose item has the same digit, among the valid ones.
+__device__ inline uint64_t digit_match(uint32_t d, bool valid) {
+  uint64_t match = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint64_t bal = __ballot((d >> b) & 1u);
+    match &= ((d >> b) & 1u) ? bal : ~bal;
+  }
+  return match;
+}
+
+// tile_hist[tile * 256 + d] = number of keys of the tile with digit d.
 template <class K>
-__global__ __launch_bounds__(kBuildBlock) void onesweep_pass(const K* __restrict__ keys_in,
-                                                             const uint32_t* __restrict__ vals_in,
-                                                             K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
-                                                             uint32_t n, int shift,
-                                                             const uint32_t* __restrict__ digit_offset,
-                                                             uint32_t* status, uint32_t* tile_counter) {
+__global__ __launch_bounds__(kBuildBlock) void sort_count(const K* __restrict__ keys_in, uint32_t n, int shift,
+                                                          uint32_t* __restrict__ tile_hist) {
+  __shared__ uint32_t s_hist[kRadix];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t first = tile * kSortTile + wave * (kSortItems * kWave) + lane;
+  K key[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = first + i * kWave;
+    key[i] = keys_in[idx < n ? idx : n - 1];
+  }
+  s_hist[tid] = 0;
+  __syncthreads();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const bool valid = first + i * kWave < n;
+    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
+    const uint64_t match = digit_match(d, valid);
+    if (valid && (match & lt_mask) == 0) atomicAdd(&s_hist[d], (uint32_t)__popcll(match));
+  }
+  __syncthreads();
+  tile_hist[(size_t)tile * kRadix + tid] = s_hist[tid];
+}
+
+// In place: tile_hist[t * 256 + d] <- digit_offset[d] + sum over tiles u < t
+// of tile_hist[u * 256 + d].  One workgroup per digit, 256 tiles per step.
+__global__ __launch_bounds__(kBuildBlock) void sort_tile_scan(uint32_t* __restrict__ tile_hist, uint32_t tiles,
+                                                              const uint32_t* __restrict__ digit_offset) {
   constexpr int kWaves = kBuildBlock / kWave;
-  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wsum[kWaves];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint32_t d = blockIdx.x;
+  uint32_t carry = digit_offset[d];
+  for (uint32_t t0 = 0; t0 < tiles; t0 += kBuildBlock) {
+    const uint32_t t = t0 + tid;
+    const uint32_t v = t < tiles ? tile_hist[(size_t)t * kRadix + d] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t s = s_wsum[w];
+      pre += w < wave ? s : 0u;
+      all += s;
+    }
+    if (t < tiles) tile_hist[(size_t)t * kRadix + d] = carry + pre + x - v;
+    carry += all;
+    __syncthreads();
+  }
+}
+
+// Ranks the tile's keys by digit, stages them in LDS in digit order and
+// writes them out in runs starting at tile_base[tile * 256 + d].
+template <class K>
+__global__ __launch_bounds__(kBuildBlock) void sort_scatter(const K* __restrict__ keys_in,
+                                                            const uint32_t* __restrict__ vals_in,
+                                                            K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                                            uint32_t n, int shift,
+                                                            const uint32_t* __restrict__ tile_base) {
+  constexpr int kWaves = kBuildBlock / kWave;
   __shared__ uint32_t s_wave_hist[kWaves][kRadix];
   __shared__ uint32_t s_local[kRadix];
   __shared__ uint32_t s_base[kRadix];
-  __shared__ uint32_t s_scan[kRadix];
+  __shared__ uint32_t s_wsum[kWaves];
   __shared__ K s_keys[kSortTile];
   __shared__ uint32_t s_vals[kSortTile];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-  for (int i = tid; i < kWaves * kRadix; i += kBuildBlock) (&s_wave_hist[0][0])[i] = 0;
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  const uint32_t tile = blockIdx.x;
   const uint32_t base = tile * kSortTile;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t first = base + wave * (kSortItems * kWave) + lane;
 
   K key[kSortItems];
   uint32_t val[kSortItems];
   uint32_t rank[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
-    const uint32_t idx = base + wave * (kSortItems * kWave) + i * kWave + lane;
-    const bool valid = idx < n;
-    key[i] = valid ? keys_in[idx] : (K)0;
-    val[i] = valid ? (vals_in ? vals_in[idx] : idx) : 0u;
-    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
-    uint64_t match = __ballot(valid);
+    const uint32_t idx = first + i * kWave;
+    key[i] = keys_in[idx < n ? idx : n - 1];
+  }
+  if (vals_in) {
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      const uint64_t bal = __ballot((d >> b) & 1u);
-      match &= ((d >> b) & 1u) ? bal : ~bal;
+    for (int i = 0; i < kSortItems; ++i) {
+      const uint32_t idx = first + i * kWave;
+      val[i] = vals_in[idx < n ? idx : n - 1];
     }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) val[i] = first + i * kWave;
+  }
+  const uint32_t my_base = tile_base[(size_t)tile * kRadix + tid];
+  for (int i = tid; i < kWaves * kRadix; i += kBuildBlock) (&s_wave_hist[0][0])[i] = 0;
+  __syncthreads();
+
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const bool valid = first + i * kWave < n;
+    const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
+    const uint64_t match = digit_match(d, valid);
     const uint32_t lower = __popcll(match & lt_mask);
     uint32_t before = 0;
     if (valid) before = s_wave_hist[wave][d];
@@ -243,51 +328,31 @@ __global__ __launch_bounds__(kBuildBlock) void onesweep_pass(const K* __restrict
   }
   __syncthreads();
 
-  // Thread t owns digit t: per-wave exclusive offsets and the tile total.
+  // Thread t owns digit t: per-wave exclusive offsets, the tile total and its
+  // exclusive scan over digits (a shuffle scan within each wave of 64 digits,
+  // then the sums of the lower waves).
   {
     const int d = tid;
     uint32_t total = 0;
+#pragma unroll
     for (int w = 0; w < kWaves; ++w) {
       const uint32_t c = s_wave_hist[w][d];
       s_wave_hist[w][d] = total;
       total += c;
     }
-    uint32_t* my_status = status + (size_t)tile * kRadix + d;
-    if (tile == 0) {
-      __hip_atomic_store(my_status, kFlagInc | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_base[d] = digit_offset[d];
-    } else {
-      __hip_atomic_store(my_status, kFlagAgg | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t excl = 0;
-      int64_t j = (int64_t)tile - 1;
-      while (j >= 0) {
-        const uint32_t s =
-            __hip_atomic_load(status + (size_t)j * kRadix + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t flag = s & ~kValueMask;
-        if (flag == 0) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        excl += s & kValueMask;
-        if (flag == kFlagInc) break;
-        --j;
-      }
-      __hip_atomic_store(my_status, kFlagInc | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_base[d] = digit_offset[d] + excl;
+    s_base[d] = my_base;
+    uint32_t x = total;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
     }
-    s_scan[d] = total;
-  }
-  __syncthreads();
-  // Exclusive scan of the tile totals over digits (Hillis-Steele in LDS).
-  {
-    const uint32_t own = s_scan[tid];
-    for (int off = 1; off < kRadix; off <<= 1) {
-      const uint32_t t = tid >= off ? s_scan[tid - off] : 0u;
-      __syncthreads();
-      s_scan[tid] += t;
-      __syncthreads();
-    }
-    s_local[tid] = s_scan[tid] - own;
+    if (lane == kWave - 1) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) pre += w < wave ? s_wsum[w] : 0u;
+    s_local[d] = pre + x - total;
   }
   __syncthreads();
 #pragma unroll
@@ -587,10 +652,7 @@ hipError_t run_sort(DeviceGrid& G, const KeyParams& kp, size_t n, hipStream_t st
   uint32_t* vals_a = reinterpret_cast<uint32_t*>(G.sort_vals_a.ptr);
   uint32_t* vals_b = reinterpret_cast<uint32_t*>(G.sort_vals_b.ptr);
   uint32_t* hist = reinterpret_cast<uint32_t*>(G.hist.ptr);
-  uint32_t* status = reinterpret_cast<uint32_t*>(G.status.ptr);
-  uint32_t* counters = hist + 8 * kRadix;
-  const size_t status_words = (size_t)kp.passes * tiles * kRadix;
-  hipLaunchKernelGGL(fill_u32, dim3(grid_for(status_words, 256, 2048)), dim3(256), 0, st, status, status_words, 0u);
+  uint32_t* tile_hist = reinterpret_cast<uint32_t*>(G.tile_hist.ptr);  // tiles x 256, reused by every pass
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, hist, (size_t)(8 * kRadix + 8), 0u);
   hipLaunchKernelGGL(keygen_hist<K>, dim3(grid_for(n, kBuildBlock, 1024)), dim3(kBuildBlock), 0, st, G.points, n, kp,
                      keys_a, hist);
@@ -601,8 +663,10 @@ hipError_t run_sort(DeviceGrid& G, const KeyParams& kp, size_t n, hipStream_t st
   K* kout = keys_b;
   uint32_t* vout = vals_b;
   for (int p = 0; p < kp.passes; ++p) {
-    hipLaunchKernelGGL(onesweep_pass<K>, dim3(tiles), dim3(kBuildBlock), 0, st, kin, vin, kout, vout, (uint32_t)n,
-                       8 * p, hist + p * kRadix, status + (size_t)p * tiles * kRadix, counters + p);
+    hipLaunchKernelGGL(sort_count<K>, dim3(tiles), dim3(kBuildBlock), 0, st, kin, (uint32_t)n, 8 * p, tile_hist);
+    hipLaunchKernelGGL(sort_tile_scan, dim3(kRadix), dim3(kBuildBlock), 0, st, tile_hist, tiles, hist + p * kRadix);
+    hipLaunchKernelGGL(sort_scatter<K>, dim3(tiles), dim3(kBuildBlock), 0, st, kin, vin, kout, vout, (uint32_t)n,
+                       8 * p, tile_hist);
     // ping-pong
     const K* nk = kout;
     const uint32_t* nv = vout;
@@ -863,7 +927,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   if ((e = G.sort_vals_a.reserve(n * 4)) != hipSuccess) return fail(e, "alloc vals");
   if ((e = G.sort_vals_b.reserve(n * 4)) != hipSuccess) return fail(e, "alloc vals");
   if ((e = G.hist.reserve((8 * kRadix + 8) * 4)) != hipSuccess) return fail(e, "alloc hist");
-  if ((e = G.status.reserve((size_t)kp.passes * tiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc status");
+  if ((e = G.tile_hist.reserve((size_t)tiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc tile table");
   if ((e = G.tile_heads.reserve((size_t)tiles * 4)) != hipSuccess) return fail(e, "alloc tiles");
   if ((e = G.seg_start.reserve(n * 4)) != hipSuccess) return fail(e, "alloc segs");
   e = G.key64 ? run_sort<uint64_t>(G, kp, n, st) : run_sort<uint32_t>(G, kp, n, st);
@@ -1094,7 +1158,7 @@ int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distanc
   if ((e = G.sort_vals_b.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc vals");
   if ((e = G.hist.reserve((8 * kRadix + 8) * 4)) != hipSuccess) return fail(e, "alloc hist");
   const uint32_t ktiles = (uint32_t)((kept + kSortTile - 1) / kSortTile);
-  if ((e = G.status.reserve((size_t)kp.passes * ktiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc status");
+  if ((e = G.tile_hist.reserve((size_t)ktiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc tile table");
   if ((e = G.tile_heads.reserve((size_t)std::max(tiles, ktiles) * 4)) != hipSuccess) return fail(e, "alloc tiles");
   if ((e = G.seg_start.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc segs");
   G.points = compact;
