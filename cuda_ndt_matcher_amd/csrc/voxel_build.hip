@@ -5,9 +5,10 @@
 // Pipeline (all device work on the handle's stream):
 //   1. key_bounds     : per-point voxel key floor(p/res) (f32 division, like
 //                       VoxelCoord::from_point); min/max over the keys.
-//   2. keygen_hist    : compact Morton code of the key relative to the box
+//   2. keygen_count   : compact Morton code of the key relative to the box
 //                       (only the significant bits of each axis), written as
-//                       u32 (<= 32 bits) or u64; per-pass digit histograms.
+//                       u32 (<= 32 bits) or u64; first-pass digit counts per
+//                       sort tile.
 //   3. radix sort     : stable LSD, 8-bit digits; per pass sort_count (digit
 //                       counts per 4096-key tile), sort_tile_scan (prefix
 //                       over tiles), sort_scatter (wave64 ballot ranking,
@@ -36,6 +37,9 @@ constexpr int kSortItems = 16;
 constexpr int kSortTile = kBuildBlock * kSortItems;  // 4096 keys per tile
 constexpr int kRadix = 256;
 static_assert(kRadix == kBuildBlock, "the sort passes give each thread one digit");
+// Row length of the digit-major per-tile count table: the tile count rounded up
+// to whole 8-entry (32-byte) groups, the unit sort_tile_scan loads.
+constexpr uint32_t tile_stride(uint32_t tiles) { return (tiles + 7u) & ~7u; }
 
 __global__ void fill_u32(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -131,53 +135,43 @@ __device__ inline int32_t key_of(float p, const KeyParams& kp) {
   return kp.mul ? mul_key(p, kp.inv) : voxel_key(p, kp.res);
 }
 
+// Keys in sort-tile order: item i of lane l in wave w of block t is point
+// t*4096 + w*1024 + i*64 + l (the layout the sort passes load with), so the
+// block also counts the first pass's digits of its tile:
+// tile_hist[d * tstride + t].  Every load is issued before any key is formed;
+// indices past n are clamped to n - 1 and their items not counted.
 template <class K>
-__global__ __launch_bounds__(kBuildBlock) void keygen_hist(const float* __restrict__ pts, size_t n, KeyParams kp,
-                                                           K* __restrict__ keys, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t s_hist[8 * kRadix];
-  for (int i = threadIdx.x; i < kp.passes * kRadix; i += kBuildBlock) s_hist[i] = 0;
+__global__ __launch_bounds__(kBuildBlock) void keygen_count(const float* __restrict__ pts, uint32_t n, KeyParams kp,
+                                                            K* __restrict__ keys, uint32_t* __restrict__ tile_hist,
+                                                            uint32_t tstride) {
+  __shared__ uint32_t s_hist[kRadix];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t first = tile * kSortTile + wave * (kSortItems * kWave) + lane;
+  float v[kSortItems][3];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = first + i * kWave;
+    const size_t c = 3 * (size_t)(idx < n ? idx : n - 1);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[i][a] = pts[c + a];
+  }
+  s_hist[tid] = 0;
   __syncthreads();
-  auto one = [&](size_t i, float x, float y, float z) {
-    const uint32_t rx = (uint32_t)(key_of(x, kp) - kp.kmin[0]);
-    const uint32_t ry = (uint32_t)(key_of(y, kp) - kp.kmin[1]);
-    const uint32_t rz = (uint32_t)(key_of(z, kp) - kp.kmin[2]);
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = first + i * kWave;
+    const uint32_t rx = (uint32_t)(key_of(v[i][0], kp) - kp.kmin[0]);
+    const uint32_t ry = (uint32_t)(key_of(v[i][1], kp) - kp.kmin[1]);
+    const uint32_t rz = (uint32_t)(key_of(v[i][2], kp) - kp.kmin[2]);
     const uint64_t m = spread_morton(rx, ry, rz, kp.layout);
-    keys[i] = (K)m;
-    for (int p = 0; p < kp.passes; ++p) atomicAdd(&s_hist[p * kRadix + ((m >> (8 * p)) & 0xFF)], 1u);
-  };
-  const size_t S = (size_t)gridDim.x * blockDim.x;
-  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (; i + 3 * S < n; i += 4 * S) {  // four points per step, loads first
-    float v[4][3];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int a = 0; a < 3; ++a) v[u][a] = pts[3 * (i + u * S) + a];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) one(i + u * S, v[u][0], v[u][1], v[u][2]);
-  }
-  for (; i < n; i += S) one(i, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
-  __syncthreads();
-  for (int i = threadIdx.x; i < kp.passes * kRadix; i += kBuildBlock)
-    if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
-}
-
-// Exclusive scan of each pass's 256-bin histogram (one workgroup).
-__global__ __launch_bounds__(kRadix) void scan_hist(uint32_t* hist, int passes) {
-  __shared__ uint32_t s[kRadix];
-  for (int p = 0; p < passes; ++p) {
-    const uint32_t v = hist[p * kRadix + threadIdx.x];
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < kRadix; off <<= 1) {
-      const uint32_t t = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
+    if (idx < n) {
+      keys[idx] = (K)m;
+      atomicAdd(&s_hist[m & 0xFFu], 1u);
     }
-    hist[p * kRadix + threadIdx.x] = s[threadIdx.x] - v;
-    __syncthreads();
   }
+  __syncthreads();
+  tile_hist[(size_t)tid * tstride + tile] = s_hist[tid];
 }
 
 // One stable LSD pass over the 8-bit digit at `shift` takes three kernels:
@@ -208,10 +202,11 @@ __device__ inline uint64_t digit_match(uint32_t d, bool valid) {
   return match;
 }
 
-// tile_hist[tile * 256 + d] = number of keys of the tile with digit d.
+// tile_hist[d * tstride + tile] = number of keys of the tile with digit d
+// (digit-major, so that sort_tile_scan reads each digit's row contiguously).
 template <class K>
 __global__ __launch_bounds__(kBuildBlock) void sort_count(const K* __restrict__ keys_in, uint32_t n, int shift,
-                                                          uint32_t* __restrict__ tile_hist) {
+                                                          uint32_t* __restrict__ tile_hist, uint32_t tstride) {
   __shared__ uint32_t s_hist[kRadix];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const uint32_t tile = blockIdx.x;
@@ -224,31 +219,47 @@ __global__ __launch_bounds__(kBuildBlock) void sort_count(const K* __restrict__ 
   }
   s_hist[tid] = 0;
   __syncthreads();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
   for (int i = 0; i < kSortItems; ++i) {
     const bool valid = first + i * kWave < n;
     const uint32_t d = (uint32_t)(key[i] >> shift) & 0xFFu;
-    const uint64_t match = digit_match(d, valid);
-    if (valid && (match & lt_mask) == 0) atomicAdd(&s_hist[d], (uint32_t)__popcll(match));
+    if (valid) atomicAdd(&s_hist[d], 1u);
   }
   __syncthreads();
-  tile_hist[(size_t)tile * kRadix + tid] = s_hist[tid];
+  tile_hist[(size_t)tid * tstride + tile] = s_hist[tid];
 }
 
-// In place: tile_hist[t * 256 + d] <- digit_offset[d] + sum over tiles u < t
-// of tile_hist[u * 256 + d].  One workgroup per digit, 256 tiles per step.
+// In place on each digit's row: tile_hist[d * tstride + t] <- sum over tiles
+// u < t of tile_hist[d * tstride + u]; digit_total[d] <- the sum over all
+// tiles.  One workgroup per digit; each thread loads eight consecutive tiles'
+// counts (two 16-byte loads; tstride is a multiple of 8 and the entries past
+// `tiles` are ignored and overwritten).
 __global__ __launch_bounds__(kBuildBlock) void sort_tile_scan(uint32_t* __restrict__ tile_hist, uint32_t tiles,
-                                                              const uint32_t* __restrict__ digit_offset) {
+                                                              uint32_t tstride, uint32_t* __restrict__ digit_total) {
   constexpr int kWaves = kBuildBlock / kWave;
+  constexpr int kPer = 8;
   __shared__ uint32_t s_wsum[kWaves];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const uint32_t d = blockIdx.x;
-  uint32_t carry = digit_offset[d];
-  for (uint32_t t0 = 0; t0 < tiles; t0 += kBuildBlock) {
-    const uint32_t t = t0 + tid;
-    const uint32_t v = t < tiles ? tile_hist[(size_t)t * kRadix + d] : 0u;
-    uint32_t x = v;
+  uint32_t* row = tile_hist + (size_t)d * tstride;
+  uint32_t carry = 0;
+  for (uint32_t t0 = 0; t0 < tstride; t0 += kPer * kBuildBlock) {
+    const uint32_t tb = t0 + tid * kPer;
+    const bool have = tb < tstride;
+    uint32_t v[kPer];
+    if (have) {
+      const uint4 lo = *reinterpret_cast<const uint4*>(row + tb);
+      const uint4 hi = *reinterpret_cast<const uint4*>(row + tb + 4);
+      v[0] = lo.x, v[1] = lo.y, v[2] = lo.z, v[3] = lo.w;
+      v[4] = hi.x, v[5] = hi.y, v[6] = hi.z, v[7] = hi.w;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      v[q] = have && tb + q < tiles ? v[q] : 0u;
+      sum += v[q];
+    }
+    uint32_t x = sum;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
       const uint32_t y = __shfl_up(x, off);
@@ -263,25 +274,39 @@ __global__ __launch_bounds__(kBuildBlock) void sort_tile_scan(uint32_t* __restri
       pre += w < wave ? s : 0u;
       all += s;
     }
-    if (t < tiles) tile_hist[(size_t)t * kRadix + d] = carry + pre + x - v;
+    if (have) {
+      uint32_t e[kPer];
+      uint32_t run = carry + pre + x - sum;
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        e[q] = run;
+        run += v[q];
+      }
+      *reinterpret_cast<uint4*>(row + tb) = make_uint4(e[0], e[1], e[2], e[3]);
+      *reinterpret_cast<uint4*>(row + tb + 4) = make_uint4(e[4], e[5], e[6], e[7]);
+    }
     carry += all;
     __syncthreads();
   }
+  if (tid == 0) digit_total[d] = carry;
 }
 
 // Ranks the tile's keys by digit, stages them in LDS in digit order and
-// writes them out in runs starting at tile_base[tile * 256 + d].
+// writes them out in runs starting at the digit's offset (exclusive scan of
+// digit_total over digits) plus tile_prefix[d * tstride + tile].
 template <class K>
 __global__ __launch_bounds__(kBuildBlock) void sort_scatter(const K* __restrict__ keys_in,
                                                             const uint32_t* __restrict__ vals_in,
                                                             K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                             uint32_t n, int shift,
-                                                            const uint32_t* __restrict__ tile_base) {
+                                                            const uint32_t* __restrict__ tile_prefix,
+                                                            uint32_t tstride,
+                                                            const uint32_t* __restrict__ digit_total) {
   constexpr int kWaves = kBuildBlock / kWave;
   __shared__ uint32_t s_wave_hist[kWaves][kRadix];
   __shared__ uint32_t s_local[kRadix];
   __shared__ uint32_t s_base[kRadix];
-  __shared__ uint32_t s_wsum[kWaves];
+  __shared__ uint32_t s_wsum[2][kWaves];
   __shared__ K s_keys[kSortTile];
   __shared__ uint32_t s_vals[kSortTile];
 
@@ -308,7 +333,8 @@ __global__ __launch_bounds__(kBuildBlock) void sort_scatter(const K* __restrict_
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) val[i] = first + i * kWave;
   }
-  const uint32_t my_base = tile_base[(size_t)tile * kRadix + tid];
+  const uint32_t my_prefix = tile_prefix[(size_t)tid * tstride + tile];
+  const uint32_t my_total = digit_total[tid];
   for (int i = tid; i < kWaves * kRadix; i += kBuildBlock) (&s_wave_hist[0][0])[i] = 0;
   __syncthreads();
 
@@ -328,9 +354,10 @@ __global__ __launch_bounds__(kBuildBlock) void sort_scatter(const K* __restrict_
   }
   __syncthreads();
 
-  // Thread t owns digit t: per-wave exclusive offsets, the tile total and its
-  // exclusive scan over digits (a shuffle scan within each wave of 64 digits,
-  // then the sums of the lower waves).
+  // Thread t owns digit t: per-wave exclusive offsets, the tile total, and
+  // the exclusive scans over digits of the tile totals and of the global
+  // totals (a shuffle scan within each wave of 64 digits, then the sums of the
+  // lower waves).
   {
     const int d = tid;
     uint32_t total = 0;
@@ -340,19 +367,29 @@ __global__ __launch_bounds__(kBuildBlock) void sort_scatter(const K* __restrict_
       s_wave_hist[w][d] = total;
       total += c;
     }
-    s_base[d] = my_base;
-    uint32_t x = total;
+    uint32_t x = total, y = my_total;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
-      const uint32_t y = __shfl_up(x, off);
-      if (lane >= off) x += y;
+      const uint32_t xs = __shfl_up(x, off);
+      const uint32_t ys = __shfl_up(y, off);
+      if (lane >= off) {
+        x += xs;
+        y += ys;
+      }
     }
-    if (lane == kWave - 1) s_wsum[wave] = x;
+    if (lane == kWave - 1) {
+      s_wsum[0][wave] = x;
+      s_wsum[1][wave] = y;
+    }
     __syncthreads();
-    uint32_t pre = 0;
+    uint32_t xpre = 0, ypre = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) pre += w < wave ? s_wsum[w] : 0u;
-    s_local[d] = pre + x - total;
+    for (int w = 0; w < kWaves; ++w) {
+      xpre += w < wave ? s_wsum[0][w] : 0u;
+      ypre += w < wave ? s_wsum[1][w] : 0u;
+    }
+    s_local[d] = xpre + x - total;
+    s_base[d] = ypre + y - my_total + my_prefix;
   }
   __syncthreads();
 #pragma unroll
@@ -651,22 +688,24 @@ hipError_t run_sort(DeviceGrid& G, const KeyParams& kp, size_t n, hipStream_t st
   K* keys_b = reinterpret_cast<K*>(G.sort_keys_b.ptr);
   uint32_t* vals_a = reinterpret_cast<uint32_t*>(G.sort_vals_a.ptr);
   uint32_t* vals_b = reinterpret_cast<uint32_t*>(G.sort_vals_b.ptr);
-  uint32_t* hist = reinterpret_cast<uint32_t*>(G.hist.ptr);
-  uint32_t* tile_hist = reinterpret_cast<uint32_t*>(G.tile_hist.ptr);  // tiles x 256, reused by every pass
-  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, st, hist, (size_t)(8 * kRadix + 8), 0u);
-  hipLaunchKernelGGL(keygen_hist<K>, dim3(grid_for(n, kBuildBlock, 1024)), dim3(kBuildBlock), 0, st, G.points, n, kp,
-                     keys_a, hist);
-  hipLaunchKernelGGL(scan_hist, dim3(1), dim3(kRadix), 0, st, hist, kp.passes);
+  uint32_t* digit_total = reinterpret_cast<uint32_t*>(G.hist.ptr);    // 256, reused by every pass
+  uint32_t* tile_hist = reinterpret_cast<uint32_t*>(G.tile_hist.ptr);  // 256 x tstride, reused by every pass
+  const uint32_t tstride = tile_stride(tiles);
+  hipLaunchKernelGGL(keygen_count<K>, dim3(tiles), dim3(kBuildBlock), 0, st, G.points, (uint32_t)n, kp, keys_a,
+                     tile_hist, tstride);
   (void)hipEventRecord(G.ev_sort0, st);
   const K* kin = keys_a;
   const uint32_t* vin = nullptr;
   K* kout = keys_b;
   uint32_t* vout = vals_b;
   for (int p = 0; p < kp.passes; ++p) {
-    hipLaunchKernelGGL(sort_count<K>, dim3(tiles), dim3(kBuildBlock), 0, st, kin, (uint32_t)n, 8 * p, tile_hist);
-    hipLaunchKernelGGL(sort_tile_scan, dim3(kRadix), dim3(kBuildBlock), 0, st, tile_hist, tiles, hist + p * kRadix);
+    if (p > 0)  // the first pass's counts come from keygen_count
+      hipLaunchKernelGGL(sort_count<K>, dim3(tiles), dim3(kBuildBlock), 0, st, kin, (uint32_t)n, 8 * p, tile_hist,
+                         tstride);
+    hipLaunchKernelGGL(sort_tile_scan, dim3(kRadix), dim3(kBuildBlock), 0, st, tile_hist, tiles, tstride,
+                       digit_total);
     hipLaunchKernelGGL(sort_scatter<K>, dim3(tiles), dim3(kBuildBlock), 0, st, kin, vin, kout, vout, (uint32_t)n,
-                       8 * p, tile_hist);
+                       8 * p, tile_hist, tstride, digit_total);
     // ping-pong
     const K* nk = kout;
     const uint32_t* nv = vout;
@@ -926,8 +965,8 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   if ((e = G.sort_keys_b.reserve(n * ksz)) != hipSuccess) return fail(e, "alloc keys");
   if ((e = G.sort_vals_a.reserve(n * 4)) != hipSuccess) return fail(e, "alloc vals");
   if ((e = G.sort_vals_b.reserve(n * 4)) != hipSuccess) return fail(e, "alloc vals");
-  if ((e = G.hist.reserve((8 * kRadix + 8) * 4)) != hipSuccess) return fail(e, "alloc hist");
-  if ((e = G.tile_hist.reserve((size_t)tiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc tile table");
+  if ((e = G.hist.reserve(kRadix * 4)) != hipSuccess) return fail(e, "alloc hist");
+  if ((e = G.tile_hist.reserve((size_t)kRadix * tile_stride(tiles) * 4)) != hipSuccess) return fail(e, "alloc tile table");
   if ((e = G.tile_heads.reserve((size_t)tiles * 4)) != hipSuccess) return fail(e, "alloc tiles");
   if ((e = G.seg_start.reserve(n * 4)) != hipSuccess) return fail(e, "alloc segs");
   e = G.key64 ? run_sort<uint64_t>(G, kp, n, st) : run_sort<uint32_t>(G, kp, n, st);
@@ -1156,9 +1195,9 @@ int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distanc
   if ((e = G.sort_keys_b.reserve((size_t)kept * ksz)) != hipSuccess) return fail(e, "alloc keys");
   if ((e = G.sort_vals_a.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc vals");
   if ((e = G.sort_vals_b.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc vals");
-  if ((e = G.hist.reserve((8 * kRadix + 8) * 4)) != hipSuccess) return fail(e, "alloc hist");
+  if ((e = G.hist.reserve(kRadix * 4)) != hipSuccess) return fail(e, "alloc hist");
   const uint32_t ktiles = (uint32_t)((kept + kSortTile - 1) / kSortTile);
-  if ((e = G.tile_hist.reserve((size_t)ktiles * kRadix * 4)) != hipSuccess) return fail(e, "alloc tile table");
+  if ((e = G.tile_hist.reserve((size_t)kRadix * tile_stride(ktiles) * 4)) != hipSuccess) return fail(e, "alloc tile table");
   if ((e = G.tile_heads.reserve((size_t)std::max(tiles, ktiles) * 4)) != hipSuccess) return fail(e, "alloc tiles");
   if ((e = G.seg_start.reserve((size_t)kept * 4)) != hipSuccess) return fail(e, "alloc segs");
   G.points = compact;
