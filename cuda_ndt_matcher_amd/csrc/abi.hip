@@ -509,7 +509,7 @@ void ndt_hip_destroy(ndt_handle* h) {
   ndt::DeviceGrid& G = h->grid;
   for (DevBuf* b : {&G.owned_points, &G.sort_keys_a, &G.sort_keys_b, &G.sort_vals_a, &G.sort_vals_b, &G.hist,
                     &G.tile_hist, &G.tile_heads, &G.seg_start, &G.scalars, &G.block_part, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
-                    &G.valid, &G.table, &G.bitmap, &G.cell_nbr, &G.cell_ent, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
+                    &G.valid, &G.table, &G.bitmap, &G.occ, &G.cell_nbr, &G.cell_ent, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
                     &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
   h->gcache.reset();

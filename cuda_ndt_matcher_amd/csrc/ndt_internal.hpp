@@ -53,7 +53,7 @@ struct DeviceGrid {
   // voxel slots (one per segment, in Morton order)
   uint32_t segments = 0;
   uint32_t num_valid = 0;
-  DevBuf vox, cov, axis, count, coord, valid, table, bitmap, cell_nbr, cell_ent;
+  DevBuf vox, cov, axis, count, coord, valid, table, bitmap, occ, cell_nbr, cell_ent;
   GridView view{};
   // host mirrors
   uint32_t* host_scalars = nullptr;  // pinned, 64 words

@@ -456,31 +456,45 @@ __global__ __launch_bounds__(1024) void seg_scan_tiles(uint32_t* tile_heads, uin
   if (threadIdx.x == 0) *total = carry;
 }
 
+// Segment starts of one tile, in position order.  Keys are loaded
+// wave-striped (item i of lane l in wave w is position tile*4096 + w*1024 +
+// i*64 + l, so (wave, item, lane) order is position order) with every load
+// issued first; a head's segment id is the tile's offset plus the heads of
+// the lower waves, the earlier items of its wave and its lower lanes.
 template <class K>
 __global__ __launch_bounds__(kBuildBlock) void seg_write(const K* __restrict__ keys, uint32_t n,
                                                          const uint32_t* __restrict__ tile_offset,
                                                          uint32_t* __restrict__ seg_start) {
-  // Each thread owns 16 consecutive elements of the tile; a block scan of the
-  // per-thread head counts gives each head its segment id.
-  __shared__ uint32_t s[kBuildBlock];
-  const uint32_t base = blockIdx.x * kSortTile + threadIdx.x * kSortItems;
-  uint32_t c = 0;
-  for (int k = 0; k < kSortItems; ++k) {
-    const uint32_t i = base + k;
-    if (i < n && (i == 0 || keys[i] != keys[i - 1])) ++c;
+  constexpr int kWaves = kBuildBlock / kWave;
+  __shared__ uint32_t s_wave[kWaves];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const uint32_t first = blockIdx.x * kSortTile + wave * (kSortItems * kWave) + lane;
+  K cur[kSortItems], prev[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = first + i * kWave;
+    const uint32_t c = idx < n ? idx : n - 1;
+    cur[i] = keys[c];
+    prev[i] = keys[c > 0 ? c - 1 : 0];
   }
-  s[threadIdx.x] = c;
+  uint64_t heads[kSortItems];
+  uint32_t count = 0;
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = first + i * kWave;
+    heads[i] = __ballot(idx < n && (idx == 0 || cur[i] != prev[i]));
+    count += (uint32_t)__popcll(heads[i]);
+  }
+  if (lane == 0) s_wave[wave] = count;
   __syncthreads();
-  for (int off = 1; off < kBuildBlock; off <<= 1) {
-    const uint32_t t = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
-  }
-  uint32_t seg = tile_offset[blockIdx.x] + s[threadIdx.x] - c;
-  for (int k = 0; k < kSortItems; ++k) {
-    const uint32_t i = base + k;
-    if (i < n && (i == 0 || keys[i] != keys[i - 1])) seg_start[seg++] = i;
+  uint32_t seg = tile_offset[blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) seg += w < wave ? s_wave[w] : 0u;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    if ((heads[i] >> lane) & 1ull) seg_start[seg + (uint32_t)__popcll(heads[i] & lt_mask)] = first + i * kWave;
+    seg += (uint32_t)__popcll(heads[i]);
   }
 }
 
@@ -611,18 +625,38 @@ __global__ __launch_bounds__(kBuildBlock) void hash_insert(const int32_t* __rest
 }
 
 // Occupancy bitmap over the padded box: bit (x - ox) + dx*((y - oy) + dy*(z - oz)).
-__global__ __launch_bounds__(kBuildBlock) void bitmap_set(const int32_t* __restrict__ coords,
-                                                          const uint32_t* __restrict__ valid, uint32_t segments,
-                                                          int32_t ox, int32_t oy, int32_t oz, uint32_t dx, uint32_t dy,
-                                                          uint32_t* bitmap) {
+// Occupied cells of the padded box as one byte per cell (occ zero-filled
+// beforehand; every writer stores the same value, so no atomics are needed),
+// then packed into the bitmap, 32 cells per word.
+__global__ __launch_bounds__(kBuildBlock) void occ_set(const int32_t* __restrict__ coords,
+                                                       const uint32_t* __restrict__ valid, uint32_t segments,
+                                                       int32_t ox, int32_t oy, int32_t oz, uint32_t dx, uint32_t dy,
+                                                       uint8_t* __restrict__ occ) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= segments || !valid[s]) return;
   const uint64_t lin = (uint64_t)(coords[3 * s] - ox) +
                        (uint64_t)dx * ((uint64_t)(coords[3 * s + 1] - oy) + (uint64_t)dy * (uint64_t)(coords[3 * s + 2] - oz));
-  atomicOr(&bitmap[lin >> 5], 1u << (lin & 31));
+  occ[lin] = 1;
 }
 
-// Cell tables: mean and slot of every valid voxel at its cell ...
+// bitmap[w] bit b = occ[32 w + b] (bytes are 0 or 1).  occ holds 32 bytes per
+// word, 16-byte aligned.
+__global__ __launch_bounds__(kBuildBlock) void bitmap_pack(const uint8_t* __restrict__ occ, size_t words,
+                                                           uint32_t* __restrict__ bitmap) {
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < words; w += (size_t)gridDim.x * blockDim.x) {
+    const uint4* q = reinterpret_cast<const uint4*>(occ + 32 * w);
+    const uint4 a = q[0], b = q[1];
+    const uint32_t part[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t bits = 0;
+    // Four 0/1 bytes to four bits: byte k (bit 8k) times 2^(21-7k) lands on bit
+    // 21+k; the sixteen partial products fall on distinct bits, so none carries.
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      bits |= ((part[j] * 0x00204081u) >> 21 & 0xFu) << (4 * j);
+    bitmap[w] = bits;
+  }
+}
+
 __global__ __launch_bounds__(kBuildBlock) void cell_ent_set(const int32_t* __restrict__ coords,
                                                             const uint32_t* __restrict__ valid,
                                                             const float4* __restrict__ vox, uint32_t segments,
@@ -636,18 +670,23 @@ __global__ __launch_bounds__(kBuildBlock) void cell_ent_set(const int32_t* __res
 
 // ... and the 27-bit block occupancy of every cell whose block lies inside
 // the padded box (zero on the outer layer, where no query block is centred).
-__global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t cells, uint32_t* cell_nbr) {
-  for (uint32_t lin = blockIdx.x * blockDim.x + threadIdx.x; lin < cells; lin += gridDim.x * blockDim.x) {
-    const uint32_t x = lin % v.bdim[0], yz = lin / v.bdim[0];
-    const uint32_t y = yz % v.bdim[1], z = yz / v.bdim[1];
+// cell_nbr over the padded box, one thread per cell: blocks walk the rows
+// (y, z) of the box, kBuildBlock cells of x at a time, so a cell's coordinates
+// need no per-thread division.
+__global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t xblocks, uint32_t* cell_nbr) {
+  const uint32_t rows = v.bdim[1] * v.bdim[2];
+  for (uint32_t blk = blockIdx.x; blk < rows * xblocks; blk += gridDim.x) {
+    const uint32_t row = blk / xblocks;
+    const uint32_t x = (blk - row * xblocks) * kBuildBlock + threadIdx.x;
+    if (x >= v.bdim[0]) continue;
+    const uint32_t y = row % v.bdim[1], z = row / v.bdim[1];
     uint32_t m = 0;
     if (x >= 1 && x + 2 <= v.bdim[0] && y >= 1 && y + 2 <= v.bdim[1] && z >= 1 && z + 2 <= v.bdim[2])
       m = occupancy27(v, (int32_t)x + v.borg[0], (int32_t)y + v.borg[1], (int32_t)z + v.borg[2]);
-    cell_nbr[lin] = m;
+    cell_nbr[(size_t)row * v.bdim[0] + x] = m;
   }
 }
 
-// Bounding box of the valid voxel coordinates (for the lookup's fast reject).
 __global__ __launch_bounds__(kBuildBlock) void valid_bounds(const int32_t* __restrict__ coords,
                                                             const uint32_t* __restrict__ valid, uint32_t segments,
                                                             int32_t* part) {
@@ -999,23 +1038,23 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   if ((e = hipMemcpyAsync(G.host_part, part, (size_t)vb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
       hipSuccess)
     return fail(e, "voxel count copy");
-  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
-  int32_t vb[6];
-  reduce_parts(vb_blocks, vb, &G.num_valid);
+  // The lookup box is the voxel-coordinate box of all points (kb), which holds
+  // every valid voxel; the valid count is read back with the final
+  // synchronisation, so the tables below are queued without a round trip.
   GridView& v = G.view;
   v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
   v.axis = reinterpret_cast<const float*>(G.axis.ptr);
   v.table = nullptr;
   v.table_mask = 0;
   for (int a = 0; a < 3; ++a) {
-    v.cmin[a] = vb[a];
-    v.cmax[a] = vb[3 + a];
+    v.cmin[a] = kb[a];
+    v.cmax[a] = kb[3 + a];
   }
   v.res = res;
   v.bitmap = nullptr;
   v.cell_nbr = nullptr;
   v.cell_ent = nullptr;
-  if (G.num_valid > 0) {
+  {
     // Occupancy bitmap over the box padded by two cells, when it has < 2^32 cells.
     uint64_t cells = 1;
     uint32_t dims[3];
@@ -1028,12 +1067,15 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
     if (cells < (1ull << 32)) {
       const size_t words = (size_t)(cells / 32) + 2;
       if ((e = G.bitmap.reserve(words * 4)) != hipSuccess) return fail(e, "alloc bitmap");
-      hipLaunchKernelGGL(fill_u32, dim3(grid_for(words, 256, 4096)), dim3(256), 0, st,
-                         reinterpret_cast<uint32_t*>(G.bitmap.ptr), words, 0u);
-      hipLaunchKernelGGL(bitmap_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+      if ((e = G.occ.reserve(words * 32)) != hipSuccess) return fail(e, "alloc bitmap");
+      hipLaunchKernelGGL(fill_u32, dim3(grid_for(words * 8, 256, 4096)), dim3(256), 0, st,
+                         reinterpret_cast<uint32_t*>(G.occ.ptr), words * 8, 0u);
+      hipLaunchKernelGGL(occ_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
                          reinterpret_cast<const int32_t*>(G.coord.ptr),
                          reinterpret_cast<const uint32_t*>(G.valid.ptr), segs, v.borg[0], v.borg[1], v.borg[2],
-                         dims[0], dims[1], reinterpret_cast<uint32_t*>(G.bitmap.ptr));
+                         dims[0], dims[1], reinterpret_cast<uint8_t*>(G.occ.ptr));
+      hipLaunchKernelGGL(bitmap_pack, dim3(grid_for(words, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st,
+                         reinterpret_cast<const uint8_t*>(G.occ.ptr), words, reinterpret_cast<uint32_t*>(G.bitmap.ptr));
       v.bitmap = reinterpret_cast<const uint32_t*>(G.bitmap.ptr);
       if (cell_tables && cells <= kCellTablesMaxCells) {
         const uint32_t nc = (uint32_t)cells;
@@ -1043,8 +1085,9 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
                            reinterpret_cast<const int32_t*>(G.coord.ptr),
                            reinterpret_cast<const uint32_t*>(G.valid.ptr), reinterpret_cast<const float4*>(G.vox.ptr),
                            segs, v, reinterpret_cast<float4*>(G.cell_ent.ptr));
-        hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for(nc, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st, v, nc,
-                           reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
+        const uint32_t xblocks = (dims[0] + kBuildBlock - 1) / kBuildBlock;
+        hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for((size_t)xblocks * dims[1] * dims[2], 1, 8192)),
+                           dim3(kBuildBlock), 0, st, v, xblocks, reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
         v.cell_nbr = reinterpret_cast<const uint32_t*>(G.cell_nbr.ptr);
         v.cell_ent = reinterpret_cast<const float4*>(G.cell_ent.ptr);
       }
@@ -1067,6 +1110,8 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   (void)hipEventRecord(G.ev_build1, st);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
   if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
+  int32_t vb[6];
+  reduce_parts(vb_blocks, vb, &G.num_valid);
   G.n_points = n;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, G.ev_build0, G.ev_build1);
