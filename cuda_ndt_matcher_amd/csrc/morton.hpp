@@ -69,6 +69,26 @@ __host__ __device__ inline uint64_t spread_by2(uint64_t v) {
   return v;
 }
 
+// bit i of v (i < 10) -> bit 3i, in 32-bit arithmetic
+__host__ __device__ inline uint32_t spread_by3_32(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// bit i of v (i < 16) -> bit 2i, in 32-bit arithmetic
+__host__ __device__ inline uint32_t spread_by2_32(uint32_t v) {
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
 __host__ __device__ inline uint32_t low_bits(uint32_t v, int n) { return n >= 32 ? v : (v & ((1u << n) - 1u)); }
 __host__ __device__ inline uint32_t axis_of(int a, uint32_t x, uint32_t y, uint32_t z) {
   return a == 0 ? x : (a == 1 ? y : z);
@@ -87,6 +107,24 @@ __host__ __device__ inline uint64_t spread_morton(uint32_t rx, uint32_t ry, uint
     pos += 2 * w;
   }
   if (L.c1 >= 0) out |= (uint64_t)low_bits(axis_of(L.c1, rx, ry, rz) >> L.m2, L.end1 - L.m2) << pos;
+  return out;
+}
+
+// spread_morton for layouts of at most 32 bits in total (then m3 <= 10, the
+// two-axis levels end below bit 32 and the one-axis part starts below it), in
+// 32-bit arithmetic.
+__host__ __device__ inline uint32_t spread_morton32(uint32_t rx, uint32_t ry, uint32_t rz, const MortonLayout& L) {
+  uint32_t out = spread_by3_32(low_bits(rx, L.m3)) | (spread_by3_32(low_bits(ry, L.m3)) << 1) |
+                 (spread_by3_32(low_bits(rz, L.m3)) << 2);
+  int pos = 3 * L.m3;
+  if (L.m2 > L.m3) {
+    const int w = L.m2 - L.m3;
+    const uint32_t a = low_bits(axis_of(L.a2, rx, ry, rz) >> L.m3, w);
+    const uint32_t b = low_bits(axis_of(L.b2, rx, ry, rz) >> L.m3, w);
+    out |= (spread_by2_32(a) | (spread_by2_32(b) << 1)) << pos;
+    pos += 2 * w;
+  }
+  if (L.c1 >= 0) out |= low_bits(axis_of(L.c1, rx, ry, rz) >> L.m2, L.end1 - L.m2) << pos;
   return out;
 }
 

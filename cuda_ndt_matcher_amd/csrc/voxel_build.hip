@@ -85,34 +85,64 @@ __device__ inline void block_minmax_store(int32_t mn[3], int32_t mx[3], uint32_t
   }
 }
 
+// Per-block box of the voxel keys.  The key sat_i32(floor(p / res)) (or
+// floor(p * inv)) never decreases as p grows (correctly rounded division or
+// product, floor and saturation all keep order), so the smallest and largest
+// keys are the keys of the smallest and largest coordinates; a NaN coordinate
+// has key 0.  Threads keep the coordinate extremes and a NaN flag per axis and
+// form six keys at the end.  With 16-byte aligned points a thread reads four
+// points at a time as three 16-byte loads, four such groups per step.
 __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restrict__ pts, size_t n, float res,
-                                                          float inv, int mul, int32_t* part) {
+                                                          float inv, int mul, int aligned16, int32_t* part) {
   auto key = [&](float p) { return mul ? sat_i32(floorf(p * inv)) : voxel_key(p, res); };
-  int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool nan[3] = {false, false, false};
+  auto take = [&](int a, float p) {
+    lo[a] = fminf(lo[a], p);  // fminf/fmaxf return the other operand for a NaN
+    hi[a] = fmaxf(hi[a], p);
+    nan[a] |= p != p;
+  };
   const size_t S = (size_t)gridDim.x * blockDim.x;
-  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  // Four points per step, all loads issued before any key: a streaming pass
-  // is bound by how many loads each wave keeps in flight.
-  for (; i + 3 * S < n; i += 4 * S) {
-    float v[4][3];
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t tail = 0;
+  if (aligned16) {
+    const float4* q = reinterpret_cast<const float4*>(pts);
+    const size_t quads = n / 4;
+    size_t c = t;
+    for (; c + 3 * S < quads; c += 4 * S) {
+      float4 f[4][3];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int a = 0; a < 3; ++a) v[u][a] = pts[3 * (i + u * S) + a];
+        for (int k = 0; k < 3; ++k) f[u][k] = q[3 * (c + u * S) + k];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 4; ++u) {
+        const float* e = reinterpret_cast<const float*>(&f[u][0]);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int32_t k = key(v[u][a]);
-        mn[a] = min(mn[a], k);
-        mx[a] = max(mx[a], k);
+        for (int k = 0; k < 12; ++k) take(k % 3, e[k]);
       }
+    }
+    for (; c < quads; c += S) {
+      float4 f[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) f[k] = q[3 * c + k];
+      const float* e = reinterpret_cast<const float*>(&f[0]);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) take(k % 3, e[k]);
+    }
+    tail = 4 * quads;
   }
-  for (; i < n; i += S) {
-    for (int a = 0; a < 3; ++a) {
-      const int32_t k = key(pts[3 * i + a]);
-      mn[a] = min(mn[a], k);
-      mx[a] = max(mx[a], k);
+  for (size_t i = tail + t; i < n; i += S)
+    for (int a = 0; a < 3; ++a) take(a, pts[3 * i + a]);
+  // A thread without points keeps +inf / -inf, whose keys are the neutral
+  // INT32_MAX / INT32_MIN.
+  int32_t mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = key(lo[a]);
+    mx[a] = key(hi[a]);
+    if (nan[a]) {
+      mn[a] = min(mn[a], 0);
+      mx[a] = max(mx[a], 0);
     }
   }
   block_minmax_store(mn, mx, 0u, part);
@@ -164,7 +194,11 @@ __global__ __launch_bounds__(kBuildBlock) void keygen_count(const float* __restr
     const uint32_t rx = (uint32_t)(key_of(v[i][0], kp) - kp.kmin[0]);
     const uint32_t ry = (uint32_t)(key_of(v[i][1], kp) - kp.kmin[1]);
     const uint32_t rz = (uint32_t)(key_of(v[i][2], kp) - kp.kmin[2]);
-    const uint64_t m = spread_morton(rx, ry, rz, kp.layout);
+    uint64_t m;
+    if constexpr (sizeof(K) == 4)  // 32-bit keys: the layout has at most 32 bits
+      m = spread_morton32(rx, ry, rz, kp.layout);
+    else
+      m = spread_morton(rx, ry, rz, kp.layout);
     if (idx < n) {
       keys[idx] = (K)m;
       atomicAdd(&s_hist[m & 0xFFu], 1u);
@@ -670,20 +704,46 @@ __global__ __launch_bounds__(kBuildBlock) void cell_ent_set(const int32_t* __res
 
 // ... and the 27-bit block occupancy of every cell whose block lies inside
 // the padded box (zero on the outer layer, where no query block is centred).
-// cell_nbr over the padded box, one thread per cell: blocks walk the rows
-// (y, z) of the box, kBuildBlock cells of x at a time, so a cell's coordinates
-// need no per-thread division.
+// cell_nbr over the padded box.  Each thread takes four consecutive cells of
+// a row (y, z): per neighbouring row it loads the six occupancy bits of
+// x0-1 .. x0+4 once and forms the four cells' 3-bit windows from them.  Cells
+// on the box's outer layer get 0 (their block would leave the box).
 __global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t xblocks, uint32_t* cell_nbr) {
   const uint32_t rows = v.bdim[1] * v.bdim[2];
+  const uint32_t quads = (v.bdim[0] + 3) / 4;
   for (uint32_t blk = blockIdx.x; blk < rows * xblocks; blk += gridDim.x) {
     const uint32_t row = blk / xblocks;
-    const uint32_t x = (blk - row * xblocks) * kBuildBlock + threadIdx.x;
-    if (x >= v.bdim[0]) continue;
+    const uint32_t q = (blk - row * xblocks) * kBuildBlock + threadIdx.x;
+    if (q >= quads) continue;
     const uint32_t y = row % v.bdim[1], z = row / v.bdim[1];
-    uint32_t m = 0;
-    if (x >= 1 && x + 2 <= v.bdim[0] && y >= 1 && y + 2 <= v.bdim[1] && z >= 1 && z + 2 <= v.bdim[2])
-      m = occupancy27(v, (int32_t)x + v.borg[0], (int32_t)y + v.borg[1], (int32_t)z + v.borg[2]);
-    cell_nbr[(size_t)row * v.bdim[0] + x] = m;
+    const uint32_t x0 = 4 * q;
+    uint32_t m[4] = {0u, 0u, 0u, 0u};
+    if (y >= 1 && y + 2 <= v.bdim[1] && z >= 1 && z + 2 <= v.bdim[2]) {
+#pragma unroll
+      for (int r = 0; r < 9; ++r) {
+        const int dy = r / 3 - 1, dz = r % 3 - 1;
+        const uint64_t lin = (uint64_t)x0 + (uint64_t)v.bdim[0] * ((uint64_t)((int32_t)y + dy) +
+                                                                   (uint64_t)v.bdim[1] * (uint64_t)((int32_t)z + dz));
+        // bit j of win: occupancy of cell x0 - 1 + j (the bit before lin = 0
+        // does not exist and reads as 0; it only serves the outer-layer cell x0 = 0)
+        const uint64_t s = lin > 0 ? lin - 1 : 0;
+        const uint64_t w = s >> 5;
+        const uint64_t two = (uint64_t)v.bitmap[w] | ((uint64_t)v.bitmap[w + 1] << 32);
+        uint32_t win = (uint32_t)(two >> (s & 31));
+        if (lin == 0) win <<= 1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const uint32_t b3 = (win >> c) & 7u;  // bit0: x-1, bit1: x, bit2: x+1
+          m[c] |= ((b3 & 1u) << r) | (((b3 >> 1) & 1u) << (9 + r)) | (((b3 >> 2) & 1u) << (18 + r));
+        }
+      }
+    }
+    uint32_t* out = cell_nbr + (size_t)row * v.bdim[0];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t x = x0 + c;
+      if (x < v.bdim[0]) out[x] = (x >= 1 && x + 2 <= v.bdim[0]) ? m[c] : 0u;
+    }
   }
 }
 
@@ -968,7 +1028,8 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
     if (count) *count = (uint32_t)c;
   };
   const int kb_blocks = grid_for(n, kBuildBlock, kMaxBoundBlocks);
-  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, 1.0f / res, 0, part);
+  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, 1.0f / res, 0,
+                     (reinterpret_cast<uintptr_t>(G.points) & 15u) == 0 ? 1 : 0, part);
   if ((e = hipMemcpyAsync(G.host_part, part, (size_t)kb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
       hipSuccess)
     return fail(e, "bounds copy");
@@ -1085,7 +1146,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
                            reinterpret_cast<const int32_t*>(G.coord.ptr),
                            reinterpret_cast<const uint32_t*>(G.valid.ptr), reinterpret_cast<const float4*>(G.vox.ptr),
                            segs, v, reinterpret_cast<float4*>(G.cell_ent.ptr));
-        const uint32_t xblocks = (dims[0] + kBuildBlock - 1) / kBuildBlock;
+        const uint32_t xblocks = ((dims[0] + 3) / 4 + kBuildBlock - 1) / kBuildBlock;
         hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for((size_t)xblocks * dims[1] * dims[2], 1, 8192)),
                            dim3(kBuildBlock), 0, st, v, xblocks, reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
         v.cell_nbr = reinterpret_cast<const uint32_t*>(G.cell_nbr.ptr);
@@ -1200,7 +1261,7 @@ int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distanc
   const float inv = 1.0f / resolution;
   const int kb_blocks = grid_for(kept, kBuildBlock, kMaxBoundBlocks);
   hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, compact, (size_t)kept, resolution, inv,
-                     1, part);
+                     1, 1, part);  // compact is a device allocation (256-byte aligned)
   if ((e = hipMemcpyAsync(G.host_part, part, (size_t)kb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
       hipSuccess)
     return fail(e, "bounds copy");

@@ -1,8 +1,9 @@
 // Host build of the voxel build's compact Morton key (csrc/morton.hpp), run by
 // tests/test_morton.py: against the definition (level i holds bit i of each
 // axis that still has bits, x, y, z order) for every width combination up to
-// 21 bits per axis, on random and all-ones coordinates.  Exit status 0 when
-// every key matches.
+// 21 bits per axis, on random and all-ones coordinates, and the 32-bit form
+// (spread_morton32) for every combination of at most 32 bits in total.  Exit
+// status 0 when every key matches.
 #include <cstdint>
 #include <cstdio>
 #include <random>
@@ -36,8 +37,13 @@ int main() {
             rz = (uint32_t)((1ull << bz) - 1);
           }
           ++n;
-          if (by_definition(rx, ry, rz, bx, by, bz) != ndt::spread_morton(rx, ry, rz, L)) {
+          const uint64_t want = by_definition(rx, ry, rz, bx, by, bz);
+          if (want != ndt::spread_morton(rx, ry, rz, L)) {
             if (bad < 5) std::printf("mismatch at widths %d %d %d\n", bx, by, bz);
+            ++bad;
+          }
+          if (bx + by + bz <= 32 && want != (uint64_t)ndt::spread_morton32(rx, ry, rz, L)) {
+            if (bad < 5) std::printf("32-bit mismatch at widths %d %d %d\n", bx, by, bz);
             ++bad;
           }
         }

@@ -1,7 +1,7 @@
 """The voxel build's compact Morton key (csrc/morton.hpp: magic-number bit
 spreading in up to three phases), built for the host from the same source,
 equals the key's definition for every per-axis width combination up to 21
-bits."""
+bits, and so does its 32-bit form for every combination of at most 32 bits."""
 import os
 import subprocess
 
