@@ -61,7 +61,7 @@ struct GridView {
   const float4* cell_ent;     // per occupied cell {mx, my, mz, slot bits}
   int32_t borg[3];          // padded box origin (cmin - 2)
   uint32_t bdim[3];         // padded box extent (cmax - cmin + 5)
-  int32_t cmin[3];          // voxel-coordinate box of the map points (holds every occupied voxel)
+  int32_t cmin[3];          // a box holding every occupied voxel (all points' voxels, or the valid ones)
   int32_t cmax[3];
   float res;
 };

@@ -1099,23 +1099,38 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   if ((e = hipMemcpyAsync(G.host_part, part, (size_t)vb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
       hipSuccess)
     return fail(e, "voxel count copy");
-  // The lookup box is the voxel-coordinate box of all points (kb), which holds
-  // every valid voxel; the valid count is read back with the final
-  // synchronisation, so the tables below are queued without a round trip.
+  // The lookup box.  When the padded voxel-coordinate box of all points (kb,
+  // which holds every valid voxel) is small enough for cell tables, it is used
+  // as is and the valid count is read back with the final synchronisation, so
+  // the tables are queued without a round trip.  Otherwise (a large map, or far
+  // outlier points whose voxels are too sparse to be valid) the build waits
+  // for the box of the valid voxels, which may still fit the cell tables.
+  auto padded_cells = [](const int32_t* box) {
+    uint64_t c = 1;
+    for (int a = 0; a < 3; ++a) c *= (uint64_t)((int64_t)box[3 + a] - box[a] + 5);
+    return c;
+  };
+  const bool box_from_keys = cell_tables && padded_cells(kb) <= kCellTablesMaxCells;
+  int32_t vb[6];
+  if (!box_from_keys) {
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
+    reduce_parts(vb_blocks, vb, &G.num_valid);
+  }
+  const int32_t* box = box_from_keys ? kb : vb;
   GridView& v = G.view;
   v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
   v.axis = reinterpret_cast<const float*>(G.axis.ptr);
   v.table = nullptr;
   v.table_mask = 0;
   for (int a = 0; a < 3; ++a) {
-    v.cmin[a] = kb[a];
-    v.cmax[a] = kb[3 + a];
+    v.cmin[a] = box[a];
+    v.cmax[a] = box[3 + a];
   }
   v.res = res;
   v.bitmap = nullptr;
   v.cell_nbr = nullptr;
   v.cell_ent = nullptr;
-  {
+  if (box_from_keys || G.num_valid > 0) {
     // Occupancy bitmap over the box padded by two cells, when it has < 2^32 cells.
     uint64_t cells = 1;
     uint32_t dims[3];
@@ -1171,8 +1186,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   (void)hipEventRecord(G.ev_build1, st);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
   if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
-  int32_t vb[6];
-  reduce_parts(vb_blocks, vb, &G.num_valid);
+  if (box_from_keys) reduce_parts(vb_blocks, vb, &G.num_valid);
   G.n_points = n;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, G.ev_build0, G.ev_build1);

@@ -152,6 +152,43 @@ def test_cell_tables_match_hash_lookups(gpu_small, small):
     assert r1.nvtl == pytest.approx(r2.nvtl, rel=1e-12)
 
 
+@pytest.mark.parametrize("far", [False, True])
+def test_sparse_outliers_leave_the_grid_unchanged(gpu_small, small, far):
+    """Isolated map points, each alone in its voxel (too few points to be valid),
+    widen the points' voxel box.  Near ones stay inside a box small enough for the
+    cell tables, which the build then uses without waiting for the valid voxels'
+    box; far ones push it past the cell-table limit, so the build waits for the
+    valid box, which still fits.  Either way the valid voxels, the lookups, the
+    derivatives and the alignment equal those of the map without them."""
+    _print_case(f"sparse outliers far={far}")
+    off = np.array([5000.0, 5000.0, 200.0]) if far else np.array([30.0, 30.0, 5.0])
+    extra = small.map_points.max(0) + off + np.array([[0.0, 0.0, 0.0], [7.0, 0.0, 0.0], [0.0, 9.0, 0.0]])
+    m = _matcher(np.concatenate([small.map_points, extra.astype(np.float32)]))
+    g0, g1 = gpu_small.target_grid(), m.target_grid()
+    assert g1["coords"].shape == g0["coords"].shape
+    o0, o1 = np.lexsort(g0["coords"].T[::-1]), np.lexsort(g1["coords"].T[::-1])
+    for k in ("coords", "means", "covs", "invs", "axes", "counts"):
+        assert np.array_equal(g0[k][o0], g1[k][o1]), k
+    rng = np.random.default_rng(5)
+    R = npref.rot_xyz(*small.guess[3:])
+    q = (small.scan.astype(np.float64) @ R.T + small.guess[:3]).astype(np.float32)
+    q = np.concatenate([q, rng.uniform(-150, 150, (300, 3)).astype(np.float32), extra.astype(np.float32)])
+    for mode in (0, 1, 2, 3):
+        c0, i0 = gpu_small.lookup(q, mode, max_k=40)
+        c1, i1 = m.lookup(q, mode, max_k=40)
+        assert np.array_equal(c0, c1), mode
+        assert _coord_sets(g0["coords"], c0, i0) == _coord_sets(g1["coords"], c1, i1), mode
+    s0, gr0, H0, n0 = gpu_small.derivatives(small.scan, small.guess, True)
+    s1, gr1, H1, n1 = m.derivatives(small.scan, small.guess, True)
+    assert n0 == n1 and s1 == pytest.approx(s0, rel=1e-12)
+    np.testing.assert_allclose(gr1, gr0, rtol=0, atol=1e-12 * np.abs(gr0).max())
+    np.testing.assert_allclose(H1, H0, rtol=0, atol=1e-12 * np.abs(H0).max())
+    r0, r1 = gpu_small.align(small.scan, small.guess), m.align(small.scan, small.guess)
+    assert r0.status == r1.status and r0.iterations == r1.iterations
+    np.testing.assert_allclose(r1.pose_vector, r0.pose_vector, rtol=0, atol=1e-9)
+    m.close()
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_derivatives_parity(gpu_small, small, oracle_grid, oracle_mod, mode):
     _print_case(f"derivatives mode {mode}")
