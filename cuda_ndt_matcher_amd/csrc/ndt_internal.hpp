@@ -52,6 +52,7 @@ struct DeviceGrid {
   int key_bits_axis[3] = {0, 0, 0};
   // voxel slots (one per segment, in Morton order)
   uint32_t segments = 0;
+  uint32_t seg_cap = 0;  // capacity of vox, cov, axis, count, coord and valid, in voxels (grow-only)
   uint32_t num_valid = 0;
   DevBuf vox, cov, axis, count, coord, valid, table, bitmap, occ, cell_nbr, cell_ent;
   GridView view{};

@@ -535,8 +535,9 @@ __global__ __launch_bounds__(kBuildBlock) void seg_write(const K* __restrict__ k
 struct StatsParams {
   float res;
   uint32_t min_points;
-  double eig_ratio;  // (double)0.01f
-  uint32_t segments;
+  double eig_ratio;                // (double)0.01f
+  const uint32_t* seg_count;       // device: number of segments
+  uint32_t cap;                    // capacity of the per-voxel arrays (segments at or past it are skipped)
   uint32_t n;
 };
 
@@ -552,9 +553,10 @@ __global__ __launch_bounds__(kBuildBlock) void voxel_stats(const float* __restri
                                                            float* __restrict__ axis_out, uint32_t* __restrict__ count_out,
                                                            int32_t* __restrict__ coord_out, uint32_t* __restrict__ valid) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= sp.segments) return;
+  const uint32_t segments = *sp.seg_count;
+  if (s >= segments || s >= sp.cap) return;
   const uint32_t b = seg_start[s];
-  const uint32_t e = (s + 1 < sp.segments) ? seg_start[s + 1] : sp.n;
+  const uint32_t e = (s + 1 < segments) ? seg_start[s + 1] : sp.n;
   const uint32_t cnt = e - b;
   const uint32_t first = order[b];
   const float p0x = pts[3 * first], p0y = pts[3 * first + 1], p0z = pts[3 * first + 2];
@@ -663,11 +665,12 @@ __global__ __launch_bounds__(kBuildBlock) void hash_insert(const int32_t* __rest
 // beforehand; every writer stores the same value, so no atomics are needed),
 // then packed into the bitmap, 32 cells per word.
 __global__ __launch_bounds__(kBuildBlock) void occ_set(const int32_t* __restrict__ coords,
-                                                       const uint32_t* __restrict__ valid, uint32_t segments,
+                                                       const uint32_t* __restrict__ valid,
+                                                       const uint32_t* __restrict__ seg_count, uint32_t cap,
                                                        int32_t ox, int32_t oy, int32_t oz, uint32_t dx, uint32_t dy,
                                                        uint8_t* __restrict__ occ) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= segments || !valid[s]) return;
+  if (s >= *seg_count || s >= cap || !valid[s]) return;
   const uint64_t lin = (uint64_t)(coords[3 * s] - ox) +
                        (uint64_t)dx * ((uint64_t)(coords[3 * s + 1] - oy) + (uint64_t)dy * (uint64_t)(coords[3 * s + 2] - oz));
   occ[lin] = 1;
@@ -693,10 +696,11 @@ __global__ __launch_bounds__(kBuildBlock) void bitmap_pack(const uint8_t* __rest
 
 __global__ __launch_bounds__(kBuildBlock) void cell_ent_set(const int32_t* __restrict__ coords,
                                                             const uint32_t* __restrict__ valid,
-                                                            const float4* __restrict__ vox, uint32_t segments,
+                                                            const float4* __restrict__ vox,
+                                                            const uint32_t* __restrict__ seg_count, uint32_t cap,
                                                             GridView v, float4* cell_ent) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= segments || !valid[s]) return;
+  if (s >= *seg_count || s >= cap || !valid[s]) return;
   const float4 m = vox[3 * s];
   cell_ent[box_index(v, coords[3 * s], coords[3 * s + 1], coords[3 * s + 2])] =
       make_float4(m.x, m.y, m.z, __uint_as_float(s));
@@ -748,10 +752,12 @@ __global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t
 }
 
 __global__ __launch_bounds__(kBuildBlock) void valid_bounds(const int32_t* __restrict__ coords,
-                                                            const uint32_t* __restrict__ valid, uint32_t segments,
+                                                            const uint32_t* __restrict__ valid,
+                                                            const uint32_t* __restrict__ seg_count, uint32_t cap,
                                                             int32_t* part) {
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
   uint32_t c = 0;
+  const uint32_t segments = min(*seg_count, cap);
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < segments; s += gridDim.x * blockDim.x) {
     const uint32_t v = valid[s];
     const int32_t k0 = coords[3 * s], k1 = coords[3 * s + 1], k2 = coords[3 * s + 2];  // loaded with valid
@@ -829,6 +835,7 @@ hipError_t run_segments(DeviceGrid& G, size_t n, hipStream_t st, uint32_t* segme
   hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_heads, tiles, scalars);
   hipLaunchKernelGGL(seg_write<K>, dim3(tiles), dim3(kBuildBlock), 0, st, keys, (uint32_t)n, tile_heads,
                      reinterpret_cast<uint32_t*>(G.seg_start.ptr));
+  if (!segments_out) return hipGetLastError();  // the count stays on the device (scalars[0])
   hipError_t e = hipMemcpyAsync(G.host_scalars, scalars, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(st);
@@ -1071,122 +1078,152 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   if ((e = G.seg_start.reserve(n * 4)) != hipSuccess) return fail(e, "alloc segs");
   e = G.key64 ? run_sort<uint64_t>(G, kp, n, st) : run_sort<uint32_t>(G, kp, n, st);
   if (e != hipSuccess) return fail(e, "radix sort");
-  uint32_t segs = 0;
-  e = G.key64 ? run_segments<uint64_t>(G, n, st, &segs) : run_segments<uint32_t>(G, n, st, &segs);
-  if (e != hipSuccess) return fail(e, "segments");
-  G.segments = segs;
-  if ((e = G.vox.reserve((size_t)segs * 48)) != hipSuccess) return fail(e, "alloc voxels");
-  if ((e = G.cov.reserve((size_t)segs * 36)) != hipSuccess) return fail(e, "alloc voxels");
-  if ((e = G.axis.reserve((size_t)segs * 12)) != hipSuccess) return fail(e, "alloc voxels");
-  if ((e = G.count.reserve((size_t)segs * 4)) != hipSuccess) return fail(e, "alloc voxels");
-  if ((e = G.coord.reserve((size_t)segs * 12)) != hipSuccess) return fail(e, "alloc voxels");
-  if ((e = G.valid.reserve((size_t)segs * 4)) != hipSuccess) return fail(e, "alloc voxels");
-  StatsParams sp;
-  sp.res = res;
-  sp.min_points = min_points;
-  sp.eig_ratio = (double)0.01f;  // ndt.rs:369 passes 0.01 as f32 (voxel_grid/types.rs:138)
-  sp.segments = segs;
-  sp.n = (uint32_t)n;
-  hipLaunchKernelGGL(voxel_stats, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st, G.points,
-                     G.sorted_vals, reinterpret_cast<const uint32_t*>(G.seg_start.ptr), sp,
-                     reinterpret_cast<float4*>(G.vox.ptr), reinterpret_cast<float*>(G.cov.ptr),
-                     reinterpret_cast<float*>(G.axis.ptr), reinterpret_cast<uint32_t*>(G.count.ptr),
-                     reinterpret_cast<int32_t*>(G.coord.ptr), reinterpret_cast<uint32_t*>(G.valid.ptr));
-  const int vb_blocks = grid_for(segs, kBuildBlock, kMaxBoundBlocks);
-  hipLaunchKernelGGL(valid_bounds, dim3(vb_blocks), dim3(kBuildBlock), 0, st,
-                     reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                     segs, part);
-  if ((e = hipMemcpyAsync(G.host_part, part, (size_t)vb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
-      hipSuccess)
-    return fail(e, "voxel count copy");
-  // The lookup box.  When the padded voxel-coordinate box of all points (kb,
-  // which holds every valid voxel) is small enough for cell tables, it is used
-  // as is and the valid count is read back with the final synchronisation, so
-  // the tables are queued without a round trip.  Otherwise (a large map, or far
-  // outlier points whose voxels are too sparse to be valid) the build waits
-  // for the box of the valid voxels, which may still fit the cell tables.
+  // Lookup box and the per-voxel arrays' capacity.  When the padded
+  // voxel-coordinate box of all points (kb, which holds every valid voxel) is
+  // small enough for cell tables, it is used as is and nothing waits for the
+  // host: the voxel kernels read the segment count from the device and the
+  // arrays keep their capacity (at least n/4 + 1024 voxels on a first build,
+  // at most n); the count and the valid count come back with the final
+  // synchronisation, and a count above the capacity grows the arrays and
+  // queues the per-voxel stage again.  Otherwise (a large map, far outlier
+  // points whose voxels are too sparse to be valid, or no cell tables) the
+  // build waits for the segment count and then for the valid voxels' box,
+  // which may still fit the cell tables, as the hash capacity and the table
+  // dimensions need them.
   auto padded_cells = [](const int32_t* box) {
     uint64_t c = 1;
     for (int a = 0; a < 3; ++a) c *= (uint64_t)((int64_t)box[3 + a] - box[a] + 5);
     return c;
   };
   const bool box_from_keys = cell_tables && padded_cells(kb) <= kCellTablesMaxCells;
+  uint32_t segs = 0;
+  e = G.key64 ? run_segments<uint64_t>(G, n, st, box_from_keys ? nullptr : &segs)
+              : run_segments<uint32_t>(G, n, st, box_from_keys ? nullptr : &segs);
+  if (e != hipSuccess) return fail(e, "segments");
+  const uint32_t* seg_count = reinterpret_cast<const uint32_t*>(G.scalars.ptr);  // written by seg_scan_tiles
   int32_t vb[6];
-  if (!box_from_keys) {
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
-    reduce_parts(vb_blocks, vb, &G.num_valid);
-  }
-  const int32_t* box = box_from_keys ? kb : vb;
-  GridView& v = G.view;
-  v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
-  v.axis = reinterpret_cast<const float*>(G.axis.ptr);
-  v.table = nullptr;
-  v.table_mask = 0;
-  for (int a = 0; a < 3; ++a) {
-    v.cmin[a] = box[a];
-    v.cmax[a] = box[3 + a];
-  }
-  v.res = res;
-  v.bitmap = nullptr;
-  v.cell_nbr = nullptr;
-  v.cell_ent = nullptr;
-  if (box_from_keys || G.num_valid > 0) {
-    // Occupancy bitmap over the box padded by two cells, when it has < 2^32 cells.
-    uint64_t cells = 1;
-    uint32_t dims[3];
-    for (int a = 0; a < 3; ++a) {
-      dims[a] = (uint32_t)((int64_t)v.cmax[a] - v.cmin[a] + 5);
-      cells *= dims[a];
-      v.borg[a] = v.cmin[a] - 2;
-      v.bdim[a] = dims[a];
+  int vb_blocks = 0;
+  // Queues voxel_stats, the valid count and the lookup structures for up to
+  // `cap` segments; 0 or a build error code.
+  auto voxel_stage = [&](uint32_t cap) -> int {
+    if ((e = G.vox.reserve((size_t)cap * 48)) != hipSuccess) return fail(e, "alloc voxels");
+    if ((e = G.cov.reserve((size_t)cap * 36)) != hipSuccess) return fail(e, "alloc voxels");
+    if ((e = G.axis.reserve((size_t)cap * 12)) != hipSuccess) return fail(e, "alloc voxels");
+    if ((e = G.count.reserve((size_t)cap * 4)) != hipSuccess) return fail(e, "alloc voxels");
+    if ((e = G.coord.reserve((size_t)cap * 12)) != hipSuccess) return fail(e, "alloc voxels");
+    if ((e = G.valid.reserve((size_t)cap * 4)) != hipSuccess) return fail(e, "alloc voxels");
+    G.seg_cap = std::max(G.seg_cap, cap);
+    const uint32_t cap_blocks = (cap + kBuildBlock - 1) / kBuildBlock;
+    StatsParams sp;
+    sp.res = res;
+    sp.min_points = min_points;
+    sp.eig_ratio = (double)0.01f;  // ndt.rs:369 passes 0.01 as f32 (voxel_grid/types.rs:138)
+    sp.seg_count = seg_count;
+    sp.cap = cap;
+    sp.n = (uint32_t)n;
+    hipLaunchKernelGGL(voxel_stats, dim3(cap_blocks), dim3(kBuildBlock), 0, st, G.points, G.sorted_vals,
+                       reinterpret_cast<const uint32_t*>(G.seg_start.ptr), sp, reinterpret_cast<float4*>(G.vox.ptr),
+                       reinterpret_cast<float*>(G.cov.ptr), reinterpret_cast<float*>(G.axis.ptr),
+                       reinterpret_cast<uint32_t*>(G.count.ptr), reinterpret_cast<int32_t*>(G.coord.ptr),
+                       reinterpret_cast<uint32_t*>(G.valid.ptr));
+    vb_blocks = grid_for(cap, kBuildBlock, kMaxBoundBlocks);
+    hipLaunchKernelGGL(valid_bounds, dim3(vb_blocks), dim3(kBuildBlock), 0, st,
+                       reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
+                       seg_count, cap, part);
+    if ((e = hipMemcpyAsync(G.host_part, part, (size_t)vb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            st)) != hipSuccess)
+      return fail(e, "voxel count copy");
+    if (!box_from_keys) {
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
+      reduce_parts(vb_blocks, vb, &G.num_valid);
     }
-    if (cells < (1ull << 32)) {
-      const size_t words = (size_t)(cells / 32) + 2;
-      if ((e = G.bitmap.reserve(words * 4)) != hipSuccess) return fail(e, "alloc bitmap");
-      if ((e = G.occ.reserve(words * 32)) != hipSuccess) return fail(e, "alloc bitmap");
-      hipLaunchKernelGGL(fill_u32, dim3(grid_for(words * 8, 256, 4096)), dim3(256), 0, st,
-                         reinterpret_cast<uint32_t*>(G.occ.ptr), words * 8, 0u);
-      hipLaunchKernelGGL(occ_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
-                         reinterpret_cast<const int32_t*>(G.coord.ptr),
-                         reinterpret_cast<const uint32_t*>(G.valid.ptr), segs, v.borg[0], v.borg[1], v.borg[2],
-                         dims[0], dims[1], reinterpret_cast<uint8_t*>(G.occ.ptr));
-      hipLaunchKernelGGL(bitmap_pack, dim3(grid_for(words, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st,
-                         reinterpret_cast<const uint8_t*>(G.occ.ptr), words, reinterpret_cast<uint32_t*>(G.bitmap.ptr));
-      v.bitmap = reinterpret_cast<const uint32_t*>(G.bitmap.ptr);
-      if (cell_tables && cells <= kCellTablesMaxCells) {
-        const uint32_t nc = (uint32_t)cells;
-        if ((e = G.cell_nbr.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
-        if ((e = G.cell_ent.reserve((size_t)nc * 16)) != hipSuccess) return fail(e, "alloc cell tables");
-        hipLaunchKernelGGL(cell_ent_set, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+    const int32_t* box = box_from_keys ? kb : vb;
+    GridView& v = G.view;
+    v.vox = reinterpret_cast<const float4*>(G.vox.ptr);
+    v.axis = reinterpret_cast<const float*>(G.axis.ptr);
+    v.table = nullptr;
+    v.table_mask = 0;
+    for (int a = 0; a < 3; ++a) {
+      v.cmin[a] = box[a];
+      v.cmax[a] = box[3 + a];
+    }
+    v.res = res;
+    v.bitmap = nullptr;
+    v.cell_nbr = nullptr;
+    v.cell_ent = nullptr;
+    if (box_from_keys || G.num_valid > 0) {
+      // Occupancy bitmap over the box padded by two cells, when it has < 2^32 cells.
+      const uint64_t cells = padded_cells(box);
+      uint32_t dims[3];
+      for (int a = 0; a < 3; ++a) {
+        dims[a] = (uint32_t)((int64_t)v.cmax[a] - v.cmin[a] + 5);
+        v.borg[a] = v.cmin[a] - 2;
+        v.bdim[a] = dims[a];
+      }
+      if (cells < (1ull << 32)) {
+        const size_t words = (size_t)(cells / 32) + 2;
+        if ((e = G.bitmap.reserve(words * 4)) != hipSuccess) return fail(e, "alloc bitmap");
+        if ((e = G.occ.reserve(words * 32)) != hipSuccess) return fail(e, "alloc bitmap");
+        hipLaunchKernelGGL(fill_u32, dim3(grid_for(words * 8, 256, 4096)), dim3(256), 0, st,
+                           reinterpret_cast<uint32_t*>(G.occ.ptr), words * 8, 0u);
+        hipLaunchKernelGGL(occ_set, dim3(cap_blocks), dim3(kBuildBlock), 0, st,
                            reinterpret_cast<const int32_t*>(G.coord.ptr),
-                           reinterpret_cast<const uint32_t*>(G.valid.ptr), reinterpret_cast<const float4*>(G.vox.ptr),
-                           segs, v, reinterpret_cast<float4*>(G.cell_ent.ptr));
-        const uint32_t xblocks = ((dims[0] + 3) / 4 + kBuildBlock - 1) / kBuildBlock;
-        hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for((size_t)xblocks * dims[1] * dims[2], 1, 8192)),
-                           dim3(kBuildBlock), 0, st, v, xblocks, reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
-        v.cell_nbr = reinterpret_cast<const uint32_t*>(G.cell_nbr.ptr);
-        v.cell_ent = reinterpret_cast<const float4*>(G.cell_ent.ptr);
+                           reinterpret_cast<const uint32_t*>(G.valid.ptr), seg_count, cap, v.borg[0], v.borg[1],
+                           v.borg[2], dims[0], dims[1], reinterpret_cast<uint8_t*>(G.occ.ptr));
+        hipLaunchKernelGGL(bitmap_pack, dim3(grid_for(words, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st,
+                           reinterpret_cast<const uint8_t*>(G.occ.ptr), words,
+                           reinterpret_cast<uint32_t*>(G.bitmap.ptr));
+        v.bitmap = reinterpret_cast<const uint32_t*>(G.bitmap.ptr);
+        if (cell_tables && cells <= kCellTablesMaxCells) {
+          const uint32_t nc = (uint32_t)cells;
+          if ((e = G.cell_nbr.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
+          if ((e = G.cell_ent.reserve((size_t)nc * 16)) != hipSuccess) return fail(e, "alloc cell tables");
+          hipLaunchKernelGGL(cell_ent_set, dim3(cap_blocks), dim3(kBuildBlock), 0, st,
+                             reinterpret_cast<const int32_t*>(G.coord.ptr),
+                             reinterpret_cast<const uint32_t*>(G.valid.ptr),
+                             reinterpret_cast<const float4*>(G.vox.ptr), seg_count, cap, v,
+                             reinterpret_cast<float4*>(G.cell_ent.ptr));
+          const uint32_t xblocks = ((dims[0] + 3) / 4 + kBuildBlock - 1) / kBuildBlock;
+          hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for((size_t)xblocks * dims[1] * dims[2], 1, 8192)),
+                             dim3(kBuildBlock), 0, st, v, xblocks, reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
+          v.cell_nbr = reinterpret_cast<const uint32_t*>(G.cell_nbr.ptr);
+          v.cell_ent = reinterpret_cast<const float4*>(G.cell_ent.ptr);
+        }
       }
     }
+    // The open-addressing hash serves the lookups only when there are no cell
+    // tables (box too large, or cell tables switched off); the segment count
+    // is known on the host then.
+    if (!v.cell_ent) {
+      const uint32_t hcap = std::max<uint32_t>(1024, next_pow2(2ull * segs));
+      if ((e = G.table.reserve((size_t)hcap * sizeof(HashEntry))) != hipSuccess) return fail(e, "alloc hash");
+      hipLaunchKernelGGL(fill_u32, dim3(grid_for((size_t)hcap * 8, 256, 4096)), dim3(256), 0, st,
+                         reinterpret_cast<uint32_t*>(G.table.ptr), (size_t)hcap * 8, 0xFFFFFFFFu);
+      hipLaunchKernelGGL(hash_insert, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
+                         reinterpret_cast<const int32_t*>(G.coord.ptr),
+                         reinterpret_cast<const uint32_t*>(G.valid.ptr), reinterpret_cast<const float4*>(G.vox.ptr),
+                         segs, reinterpret_cast<HashEntry*>(G.table.ptr), hcap - 1);
+      v.table = reinterpret_cast<const HashEntry*>(G.table.ptr);
+      v.table_mask = hcap - 1;
+    }
+    if (box_from_keys &&
+        (e = hipMemcpyAsync(G.host_scalars, seg_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return fail(e, "segment count copy");
+    (void)hipEventRecord(G.ev_build1, st);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
+    return 0;
+  };
+  const uint32_t cap0 =
+      box_from_keys ? (uint32_t)std::min<size_t>(n, std::max<size_t>(G.seg_cap, n / 4 + 1024)) : segs;
+  int rc = voxel_stage(cap0);
+  if (rc) return rc;
+  if (box_from_keys) {
+    segs = G.host_scalars[0];
+    if (segs > cap0 && (rc = voxel_stage(segs)) != 0) return rc;
+    reduce_parts(vb_blocks, vb, &G.num_valid);
   }
-  // The open-addressing hash serves the lookups only when there are no cell
-  // tables (box too large, or cell tables switched off).
-  if (!v.cell_ent) {
-    const uint32_t cap = std::max<uint32_t>(1024, next_pow2(2ull * segs));
-    if ((e = G.table.reserve((size_t)cap * sizeof(HashEntry))) != hipSuccess) return fail(e, "alloc hash");
-    hipLaunchKernelGGL(fill_u32, dim3(grid_for((size_t)cap * 8, 256, 4096)), dim3(256), 0, st,
-                       reinterpret_cast<uint32_t*>(G.table.ptr), (size_t)cap * 8, 0xFFFFFFFFu);
-    hipLaunchKernelGGL(hash_insert, dim3((segs + kBuildBlock - 1) / kBuildBlock), dim3(kBuildBlock), 0, st,
-                       reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                       reinterpret_cast<const float4*>(G.vox.ptr), segs, reinterpret_cast<HashEntry*>(G.table.ptr),
-                       cap - 1);
-    v.table = reinterpret_cast<const HashEntry*>(G.table.ptr);
-    v.table_mask = cap - 1;
-  }
-  (void)hipEventRecord(G.ev_build1, st);
-  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
-  if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
-  if (box_from_keys) reduce_parts(vb_blocks, vb, &G.num_valid);
+  G.segments = segs;
   G.n_points = n;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, G.ev_build0, G.ev_build1);

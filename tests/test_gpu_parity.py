@@ -189,6 +189,40 @@ def test_sparse_outliers_leave_the_grid_unchanged(gpu_small, small, far):
     m.close()
 
 
+def test_voxel_arrays_grow_when_the_segment_count_exceeds_the_first_estimate(gpu_small, small):
+    """A fresh handle sizes its per-voxel arrays for n/4 + 1024 voxels without
+    waiting for the segment count.  A lattice of single points above the map (one
+    per cell: too few to be valid) pushes the count past that, so the build grows
+    the arrays and runs the per-voxel stage again; a second build on the same
+    handle fits at once.  Both give the map's own voxels, lookups and alignment."""
+    _print_case("voxel arrays regrow")
+    i, j, l = np.meshgrid(np.arange(100), np.arange(100), np.arange(4), indexing="ij")
+    lattice = np.stack([-99.0 + 2 * i, -99.0 + 2 * j, 31.0 + 4 * l], -1).reshape(-1, 3).astype(np.float32)
+    pts = np.concatenate([small.map_points, lattice])
+    keys = np.floor(pts / np.float32(2.0)).astype(np.int64)
+    segments = np.unique(keys, axis=0).shape[0]
+    assert segments > pts.shape[0] // 4 + 1024  # the premise: the first estimate is too small
+    g0 = gpu_small.target_grid()
+    o0 = np.lexsort(g0["coords"].T[::-1])
+    r0 = gpu_small.align(small.scan, small.guess)
+    m = _matcher(pts)
+    for build in range(2):
+        if build:
+            m.set_target(pts)
+        g1 = m.target_grid()
+        o1 = np.lexsort(g1["coords"].T[::-1])
+        for k in ("coords", "means", "covs", "invs", "axes", "counts"):
+            assert np.array_equal(g0[k][o0], g1[k][o1]), (build, k)
+        c0, i0 = gpu_small.lookup(small.scan, 0, max_k=40)
+        c1, i1 = m.lookup(small.scan, 0, max_k=40)
+        assert np.array_equal(c0, c1)
+        assert _coord_sets(g0["coords"], c0, i0) == _coord_sets(g1["coords"], c1, i1)
+        r1 = m.align(small.scan, small.guess)
+        assert r0.status == r1.status and r0.iterations == r1.iterations
+        np.testing.assert_allclose(r1.pose_vector, r0.pose_vector, rtol=0, atol=1e-9)
+    m.close()
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_derivatives_parity(gpu_small, small, oracle_grid, oracle_mod, mode):
     _print_case(f"derivatives mode {mode}")
