@@ -486,7 +486,8 @@ int ndt_hip_create(const ndt_config* cfg, ndt_handle** out) {
   if (e == hipSuccess) e = hipEventCreate(&h->grid.ev_sort0);
   if (e == hipSuccess) e = hipEventCreate(&h->grid.ev_sort1);
   if (e == hipSuccess)
-    e = hipHostMalloc(reinterpret_cast<void**>(&h->grid.host_scalars), 64 * sizeof(uint32_t), hipHostMallocDefault);
+    e = hipHostMalloc(reinterpret_cast<void**>(&h->grid.host_scalars), 64 * sizeof(uint32_t),
+                      hipHostMallocCoherent | hipHostMallocMapped);  // written by kernels
   if (e != hipSuccess) {
     ndt_hip_destroy(h);
     return NDT_ERR_HIP;
@@ -508,14 +509,14 @@ void ndt_hip_destroy(ndt_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   ndt::DeviceGrid& G = h->grid;
   for (DevBuf* b : {&G.owned_points, &G.sort_keys_a, &G.sort_keys_b, &G.sort_vals_a, &G.sort_vals_b, &G.hist,
-                    &G.tile_hist, &G.tile_heads, &G.seg_start, &G.scalars, &G.block_part, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
+                    &G.tile_hist, &G.tile_heads, &G.seg_start, &G.scalars, &G.vox, &G.cov, &G.axis, &G.count, &G.coord,
                     &G.valid, &G.table, &G.bitmap, &G.occ, &G.cell_nbr, &G.cell_ent, &h->scan, &h->jobs, &h->results, &h->ws, &h->poses, &h->partials, &h->out,
                     &h->lookup_q, &h->lookup_c, &h->lookup_i, &h->phase})
     b->release();
   h->gcache.reset();
   ndt::DeviceGrid& F = h->filt;
   for (DevBuf* b : {&F.owned_points, &F.sort_keys_a, &F.sort_keys_b, &F.sort_vals_a, &F.sort_vals_b, &F.hist,
-                    &F.tile_hist, &F.tile_heads, &F.seg_start, &F.scalars, &F.block_part, &h->filt_in, &h->filt_out})
+                    &F.tile_hist, &F.tile_heads, &F.seg_start, &F.scalars, &h->filt_in, &h->filt_out})
     b->release();
   for (ndt::DeviceGrid* g : {&G, &F}) {
     if (g->host_scalars) (void)hipHostFree(g->host_scalars);

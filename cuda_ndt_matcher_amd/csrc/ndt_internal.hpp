@@ -58,8 +58,8 @@ struct DeviceGrid {
   GridView view{};
   // host mirrors
   uint32_t* host_scalars = nullptr;  // pinned, 64 words
-  DevBuf block_part;                 // per-workgroup bounds partials (8 words each)
-  int32_t* host_part = nullptr;      // pinned mirror of block_part
+  int32_t* host_part = nullptr;      // per-workgroup bounds partials (8 words each), written by the
+                                     // kernels into pinned, coherent, mapped host memory
   hipEvent_t ev_build0{}, ev_build1{}, ev_sort0{}, ev_sort1{};
   float build_ms = 0.f, sort_ms = 0.f;
   bool ready = false;

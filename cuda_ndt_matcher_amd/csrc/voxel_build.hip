@@ -468,7 +468,9 @@ __global__ __launch_bounds__(kBuildBlock) void seg_count(const K* __restrict__ k
   }
 }
 
-__global__ __launch_bounds__(1024) void seg_scan_tiles(uint32_t* tile_heads, uint32_t tiles, uint32_t* total) {
+// The segment count goes to *total (device) and *host_total (pinned host memory).
+__global__ __launch_bounds__(1024) void seg_scan_tiles(uint32_t* tile_heads, uint32_t tiles, uint32_t* total,
+                                                       uint32_t* host_total) {
   __shared__ uint32_t s[1024];
   uint32_t carry = 0;
   for (uint32_t base = 0; base < tiles; base += 1024) {
@@ -487,7 +489,10 @@ __global__ __launch_bounds__(1024) void seg_scan_tiles(uint32_t* tile_heads, uin
     __syncthreads();
     carry += block_total;
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) {
+    *total = carry;
+    *host_total = carry;
+  }
 }
 
 // Segment starts of one tile, in position order.  Keys are loaded
@@ -832,13 +837,11 @@ hipError_t run_segments(DeviceGrid& G, size_t n, hipStream_t st, uint32_t* segme
   uint32_t* tile_heads = reinterpret_cast<uint32_t*>(G.tile_heads.ptr);
   uint32_t* scalars = reinterpret_cast<uint32_t*>(G.scalars.ptr);
   hipLaunchKernelGGL(seg_count<K>, dim3(tiles), dim3(kBuildBlock), 0, st, keys, (uint32_t)n, tile_heads);
-  hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_heads, tiles, scalars);
+  hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_heads, tiles, scalars, G.host_scalars);
   hipLaunchKernelGGL(seg_write<K>, dim3(tiles), dim3(kBuildBlock), 0, st, keys, (uint32_t)n, tile_heads,
                      reinterpret_cast<uint32_t*>(G.seg_start.ptr));
-  if (!segments_out) return hipGetLastError();  // the count stays on the device (scalars[0])
-  hipError_t e = hipMemcpyAsync(G.host_scalars, scalars, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-  if (e != hipSuccess) return e;
-  e = hipStreamSynchronize(st);
+  if (!segments_out) return hipGetLastError();  // read from host_scalars after a later synchronisation
+  hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) return e;
   *segments_out = G.host_scalars[0];
   return hipGetLastError();
@@ -1010,14 +1013,12 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   int32_t* bounds = reinterpret_cast<int32_t*>(G.scalars.ptr) + 8;
   uint32_t* extra = reinterpret_cast<uint32_t*>(G.scalars.ptr);
   hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra, 8);
-  if ((e = G.block_part.reserve((size_t)kMaxBoundBlocks * 8 * sizeof(int32_t))) != hipSuccess)
-    return fail(e, "alloc");
   if (!G.host_part &&
       (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_part), (size_t)kMaxBoundBlocks * 8 * sizeof(int32_t),
-                         hipHostMallocDefault)) != hipSuccess)
+                         hipHostMallocCoherent | hipHostMallocMapped)) != hipSuccess)
     return fail(e, "pinned alloc");
-  int32_t* part = reinterpret_cast<int32_t*>(G.block_part.ptr);
-  // per-workgroup partials reduced on the host (n > 0, so every block has an entry)
+  // per-workgroup partials, written by the kernels into host_part and reduced on
+  // the host (n > 0, so every block has an entry)
   auto reduce_parts = [&](int blocks, int32_t out[6], uint32_t* count) {
     for (int a = 0; a < 3; ++a) {
       out[a] = INT32_MAX;
@@ -1035,11 +1036,10 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
     if (count) *count = (uint32_t)c;
   };
   const int kb_blocks = grid_for(n, kBuildBlock, kMaxBoundBlocks);
+  // The per-workgroup partials go straight to pinned host memory (coherent and
+  // mapped), so the stream carries no copy ahead of the synchronisation.
   hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, 1.0f / res, 0,
-                     (reinterpret_cast<uintptr_t>(G.points) & 15u) == 0 ? 1 : 0, part);
-  if ((e = hipMemcpyAsync(G.host_part, part, (size_t)kb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
-      hipSuccess)
-    return fail(e, "bounds copy");
+                     (reinterpret_cast<uintptr_t>(G.points) & 15u) == 0 ? 1 : 0, G.host_part);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "bounds sync");
   int32_t kb[6];
   reduce_parts(kb_blocks, kb, nullptr);
@@ -1129,10 +1129,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
     vb_blocks = grid_for(cap, kBuildBlock, kMaxBoundBlocks);
     hipLaunchKernelGGL(valid_bounds, dim3(vb_blocks), dim3(kBuildBlock), 0, st,
                        reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                       seg_count, cap, part);
-    if ((e = hipMemcpyAsync(G.host_part, part, (size_t)vb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost,
-                            st)) != hipSuccess)
-      return fail(e, "voxel count copy");
+                       seg_count, cap, G.host_part);
     if (!box_from_keys) {
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
       reduce_parts(vb_blocks, vb, &G.num_valid);
@@ -1206,9 +1203,6 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
       v.table = reinterpret_cast<const HashEntry*>(G.table.ptr);
       v.table_mask = hcap - 1;
     }
-    if (box_from_keys &&
-        (e = hipMemcpyAsync(G.host_scalars, seg_count, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
-      return fail(e, "segment count copy");
     (void)hipEventRecord(G.ev_build1, st);
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
     if ((e = hipGetLastError()) != hipSuccess) return fail(e, "build kernels");
@@ -1258,12 +1252,13 @@ int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distanc
   }
   hipError_t e;
   if (!G.host_scalars &&
-      (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_scalars), 64 * sizeof(uint32_t), hipHostMallocDefault)) !=
+      (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_scalars), 64 * sizeof(uint32_t),
+                         hipHostMallocCoherent | hipHostMallocMapped)) !=
           hipSuccess)
     return fail(e, "pinned alloc");
   if (!G.host_part &&
       (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_part), (size_t)kMaxBoundBlocks * 8 * sizeof(int32_t),
-                         hipHostMallocDefault)) != hipSuccess)
+                         hipHostMallocCoherent | hipHostMallocMapped)) != hipSuccess)
     return fail(e, "pinned alloc");
   const uint32_t tiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
   if ((e = G.scalars.reserve(64 * sizeof(uint32_t))) != hipSuccess) return fail(e, "alloc");
@@ -1279,7 +1274,7 @@ int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distanc
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, st, scalars, (size_t)64, 0u);
   hipLaunchKernelGGL(filter_count, dim3(tiles), dim3(kBuildBlock), 0, st, G.points, (uint32_t)n, f, tile_kept,
                      scalars);
-  hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_kept, tiles, scalars);
+  hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_kept, tiles, scalars, G.host_scalars);
   if ((e = hipMemcpyAsync(G.host_scalars, scalars, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
     return fail(e, "count copy");
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "filter sync");
@@ -1306,16 +1301,10 @@ int filter_points(DeviceGrid& G, size_t n, float min_distance, float max_distanc
   hipLaunchKernelGGL(filter_compact, dim3(tiles), dim3(kBuildBlock), 0, st, G.points, (uint32_t)n, f, tile_kept,
                      compact);
   // cells of the kept points, as voxel_downsample_cpu forms them
-  if ((e = G.block_part.reserve((size_t)kMaxBoundBlocks * 8 * sizeof(int32_t))) != hipSuccess)
-    return fail(e, "alloc");
-  int32_t* part = reinterpret_cast<int32_t*>(G.block_part.ptr);
   const float inv = 1.0f / resolution;
   const int kb_blocks = grid_for(kept, kBuildBlock, kMaxBoundBlocks);
   hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, compact, (size_t)kept, resolution, inv,
-                     1, 1, part);  // compact is a device allocation (256-byte aligned)
-  if ((e = hipMemcpyAsync(G.host_part, part, (size_t)kb_blocks * 8 * sizeof(int32_t), hipMemcpyDeviceToHost, st)) !=
-      hipSuccess)
-    return fail(e, "bounds copy");
+                     1, 1, G.host_part);  // compact is a device allocation (256-byte aligned)
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "bounds sync");
   int32_t kb[6];
   for (int a = 0; a < 3; ++a) {
