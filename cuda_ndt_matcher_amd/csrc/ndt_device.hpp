@@ -604,52 +604,6 @@ __device__ inline void block_sums_lds(const double* v, int count, double* red, d
 }
 
 // ---------------------------------------------------------------------------
-// Symmetric eigendecomposition, cyclic Jacobi, FP64 (device).  Reads the lower
-// triangle.  v is row-major with eigenvectors as columns.
-template <int N>
-__device__ inline void sym_eigen(const double* a_in, double* ev, double* v) {
-  double a[N][N];
-  for (int i = 0; i < N; ++i)
-    for (int j = 0; j <= i; ++j) a[i][j] = a[j][i] = a_in[i * N + j];
-  for (int i = 0; i < N; ++i)
-    for (int j = 0; j < N; ++j) v[i * N + j] = (i == j) ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int i = 0; i < N; ++i) {
-      diag += a[i][i] * a[i][i];
-      for (int j = i + 1; j < N; ++j) off += a[i][j] * a[i][j];
-    }
-    if (off == 0.0 || off <= 1e-34 * diag) break;
-    for (int p = 0; p < N - 1; ++p)
-      for (int q = p + 1; q < N; ++q) {
-        const double apq = a[p][q];
-        if (apq == 0.0) continue;
-        const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
-        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0);
-        const double s = t * c;
-        for (int k = 0; k < N; ++k) {
-          const double akp = a[k][p], akq = a[k][q];
-          a[k][p] = c * akp - s * akq;
-          a[k][q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < N; ++k) {
-          const double apk = a[p][k], aqk = a[q][k];
-          a[p][k] = c * apk - s * aqk;
-          a[q][k] = s * apk + c * aqk;
-        }
-        a[p][q] = a[q][p] = 0.0;
-        for (int k = 0; k < N; ++k) {
-          const double vkp = v[k * N + p], vkq = v[k * N + q];
-          v[k * N + p] = c * vkp - s * vkq;
-          v[k * N + q] = s * vkp + c * vkq;
-        }
-      }
-  }
-  for (int i = 0; i < N; ++i) ev[i] = a[i][i];
-}
-
-// ---------------------------------------------------------------------------
 // Workgroup-cooperative 6x6 Newton step.
 //
 // newton_step_negative_definite (optimization/newton.rs:160-176): when every

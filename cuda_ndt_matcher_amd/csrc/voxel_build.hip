@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 
+#include "eigen3.hpp"
 #include "morton.hpp"
 #include "ndt_device.hpp"
 #include "ndt_internal.hpp"
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(kBuildBlock) void voxel_stats(const float* __restri
   c[2] = c[6] = (sxz - sx * mz) * inv_n1;
   c[5] = c[7] = (syz - sy * mz) * inv_n1;
   double ev[3], V[9];
-  sym_eigen<3>(c, ev, V);
+  sym_eigen3<DeviceEigenMath>(c, ev, V);
   const double max_ev = fmax(fmax(fmax(0.0, ev[0]), ev[1]), ev[2]);
   if (!(max_ev > 0.0)) return;  // types.rs:126-128 (NaN statistics are dropped too)
   int min_idx = 0;

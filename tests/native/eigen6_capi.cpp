@@ -1,7 +1,9 @@
-// Host build of the Newton solve's eigendecomposition (csrc/eigen6.hpp) for
-// the CPU tests: the same source the device compiles, called once per row.
-// Built with -D__host__= -D__device__= by tests/test_eigen6.py.
+// Host build of the Newton solve's eigendecomposition (csrc/eigen6.hpp) and
+// of the voxel covariances' 3x3 Jacobi (csrc/eigen3.hpp) for the CPU tests:
+// the same sources the device compiles.  Built with -D__host__= -D__device__=
+// by tests/test_eigen6.py and tests/test_eigen3.py.
 #include "../../cuda_ndt_matcher_amd/csrc/eigen6.hpp"
+#include "../../cuda_ndt_matcher_amd/csrc/eigen3.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -60,6 +62,15 @@ int full(const double* H, const double* b, double* lam, double* Q, double* y) {
 // mode 0: exact arithmetic; 1: the device's estimate-and-refine form.
 extern "C" int eigen6_full(const double* H, const double* b, double* lam, double* Q, double* y, int mode) {
   return mode ? full<EstimateMath>(H, b, lam, Q, y) : full<HostMath>(H, b, lam, Q, y);
+}
+
+// 3x3: a row-major (lower triangle read), ev eigenvalues, v eigenvectors as
+// columns; mode as above.
+extern "C" void sym3_full(const double* a, double* ev, double* v, int mode) {
+  if (mode)
+    ndt::sym_eigen3<EstimateMath>(a, ev, v);
+  else
+    ndt::sym_eigen3<HostMath>(a, ev, v);
 }
 
 extern "C" double refine_rsqrt_from(double x, double y0) { return ndt::refine_rsqrt(x, y0); }
