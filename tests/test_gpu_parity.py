@@ -102,6 +102,33 @@ def test_build_parity_small(gpu_small, small, oracle_grid, oracle_mod):
     _check_build(gpu_small, small.map_points, oracle_grid, oracle_mod)
 
 
+def test_build_parity_wide_keys(small, oracle_mod):
+    """Two copies of the small map 4.1 km apart on every axis: 12 key bits per
+    axis, 36 in all, so the build sorts 64-bit keys in five passes; the box is
+    far past the cell tables, so the lookups take the hash.  Keys, order,
+    counts and statistics as the oracle's; lookups around both copies as the
+    oracle's search."""
+    _print_case("build parity 64-bit keys")
+    off = np.float32(4100.0)
+    pts = np.concatenate([small.map_points, small.map_points + off]).astype(np.float32)
+    span = np.ptp(np.floor(pts / np.float32(2.0)).astype(np.int64), axis=0)
+    assert sum(int(s).bit_length() for s in span) > 32
+    grid = oracle_mod.Grid(pts, 2.0)
+    m = _matcher(pts)
+    _check_build(m, pts, grid, oracle_mod)
+    rng = np.random.default_rng(21)
+    q = small.scan.astype(np.float32)[:400]
+    q = np.concatenate([q, q + off, rng.uniform(-100, 100, (200, 3)).astype(np.float32) + off])
+    g = m.target_grid()
+    ex = grid.export()
+    for mode in (0, 1):
+        c, i = m.lookup(q, mode, max_k=40)
+        co, io = grid.search(q, mode, max_k=40)
+        assert np.array_equal(c, co), mode
+        assert _coord_sets(g["coords"], c, i) == _coord_sets(ex["coords"], co, io), mode
+    m.close()
+
+
 def _coord_sets(coords, cnt, idx):
     out = []
     for i in range(len(cnt)):
