@@ -164,8 +164,8 @@ void ndt_hip_destroy(ndt_handle* h);
 const char* ndt_hip_last_error(const ndt_handle* h);
 
 /* NdtScanMatcher::set_target (ndt.rs:361-415).  Builds the voxel grid on the
- * GPU (Morton keys, LDS onesweep radix sort, segmented moments, eigen
- * regularisation, voxel hash).  Replaces the CUB bindings of cuda_ffi
+ * GPU (Morton keys, LDS-staged radix sort, segmented moments, eigen
+ * regularisation, cell tables or voxel hash).  Replaces the CUB bindings of cuda_ffi
  * (radix_sort.cu:24,62; segment_detect.cu; voxel_hash.cu:218-386). */
 int ndt_hip_set_target(ndt_handle* h, const float* xyz, size_t n);
 int ndt_hip_set_target_device(ndt_handle* h, const float* d_xyz, size_t n);
