@@ -102,6 +102,21 @@ def test_build_parity_small(gpu_small, small, oracle_grid, oracle_mod):
     _check_build(gpu_small, small.map_points, oracle_grid, oracle_mod)
 
 
+@pytest.mark.parametrize("n", [6, 100, 4095, 4096, 4097, 12289])
+def test_build_parity_tile_edges(oracle_mod, n):
+    """Maps whose point counts sit on and around the 4096-point sort tiles (a
+    single partial tile, exact tiles, one point past a tile): keys, order,
+    counts and statistics as the oracle's."""
+    _print_case(f"build parity n={n}")
+    rng = np.random.default_rng(n)
+    # six points inside one voxel; about 25 per voxel at n = 100; then a 6 x 5 x 2 voxel box
+    extent = [1.5, 1.5, 1.5] if n < 50 else ([4.0, 4.0, 2.0] if n <= 1000 else [12.0, 10.0, 4.0])
+    pts = (rng.random((n, 3)) * np.array(extent) + [100.2, -51.8, 2.1]).astype(np.float32)
+    m = _matcher(pts)
+    _check_build(m, pts, oracle_mod.Grid(pts, 2.0), oracle_mod)
+    m.close()
+
+
 def test_build_parity_wide_keys(small, oracle_mod):
     """Two copies of the small map 4.1 km apart on every axis: 12 key bits per
     axis, 36 in all, so the build sorts 64-bit keys in five passes; the box is
