@@ -187,7 +187,8 @@ int ensure_host_results(ndt_handle* h, size_t m) {
   if (m <= h->host_results_cap) return NDT_OK;
   if (h->host_results) (void)hipHostFree(h->host_results);
   h->host_results = nullptr;
-  NDT_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->host_results), m * sizeof(ndt_result), hipHostMallocDefault),
+  NDT_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->host_results), m * sizeof(ndt_result),
+                        hipHostMallocCoherent | hipHostMallocMapped),  // written by the Newton kernel
           "pinned alloc");
   h->host_results_cap = m;
   return NDT_OK;
@@ -257,17 +258,14 @@ int launch_once(ndt_handle* h, size_t m, int W, bool allow_graph, float* ms, boo
   } else {
     if (h->cfg.profile_phases)
       NDT_HIP(hipMemsetAsync(h->phase.ptr, 0, 16 * sizeof(unsigned long long), h->stream), "clear phase");
-    NDT_HIP(hipMemcpyAsync(h->jobs.ptr, h->host_jobs, m * sizeof(ndt::SlotJob), hipMemcpyHostToDevice, h->stream),
-            "copy jobs");
-    if (hipError_t e = ndt::launch_newton(h->grid.view, cfg, reinterpret_cast<const ndt::SlotJob*>(h->jobs.ptr),
-                                          (int)m, reinterpret_cast<ndt_result*>(h->results.ptr), h->ws.ptr, h->stream,
-                                          W > 1, h->ev0);
+    // The kernel reads the jobs from and writes the results to the pinned,
+    // mapped host buffers: no copies around the launch (each would add a
+    // queue round trip to a launch of well under a millisecond).
+    if (hipError_t e = ndt::launch_newton(h->grid.view, cfg, h->host_jobs, (int)m, h->host_results, h->ws.ptr,
+                                          h->stream, W > 1, h->ev0);
         e != hipSuccess)
       return launch_err(h, e, "newton launch", W > 1);
     NDT_HIP(hipEventRecord(h->ev1, h->stream), "event");
-    NDT_HIP(hipMemcpyAsync(h->host_results, h->results.ptr, m * sizeof(ndt_result), hipMemcpyDeviceToHost,
-                           h->stream),
-            "copy results");
   }
   NDT_HIP(hipStreamSynchronize(h->stream), "newton sync");
   NDT_HIP(hipEventElapsedTime(ms, h->ev0, h->ev1), "event time");
@@ -326,7 +324,8 @@ int run_alignments(ndt_handle* h, const float* const* d_pts, const size_t* n, co
     if (h->host_jobs) (void)hipHostFree(h->host_jobs);
     h->host_jobs = nullptr;
     h->host_jobs_cap = 0;
-    NDT_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->host_jobs), m * sizeof(ndt::SlotJob), hipHostMallocDefault),
+    NDT_HIP(hipHostMalloc(reinterpret_cast<void**>(&h->host_jobs), m * sizeof(ndt::SlotJob),
+                          hipHostMallocCoherent | hipHostMallocMapped),  // read by the Newton kernel
             "pinned alloc");
     h->host_jobs_cap = m;
   }
