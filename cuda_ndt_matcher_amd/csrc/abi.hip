@@ -520,6 +520,7 @@ void ndt_hip_destroy(ndt_handle* h) {
   for (ndt::DeviceGrid* g : {&G, &F}) {
     if (g->host_scalars) (void)hipHostFree(g->host_scalars);
     if (g->host_part) (void)hipHostFree(g->host_part);
+    if (g->host_valid) (void)hipHostFree(g->host_valid);
   }
   for (hipEvent_t ev : {F.ev_sort0, F.ev_sort1})
     if (ev) (void)hipEventDestroy(ev);

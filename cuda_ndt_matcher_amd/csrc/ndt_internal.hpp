@@ -60,6 +60,8 @@ struct DeviceGrid {
   uint32_t* host_scalars = nullptr;  // pinned, 64 words
   int32_t* host_part = nullptr;      // per-workgroup bounds partials (8 words each), written by the
                                      // kernels into pinned, coherent, mapped host memory
+  uint32_t* host_valid = nullptr;    // per-workgroup valid-voxel counts (occ_set), pinned and mapped
+  uint32_t host_valid_cap = 0;       // entries of host_valid
   hipEvent_t ev_build0{}, ev_build1{}, ev_sort0{}, ev_sort1{};
   float build_ms = 0.f, sort_ms = 0.f;
   bool ready = false;

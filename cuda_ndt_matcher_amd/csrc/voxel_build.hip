@@ -670,16 +670,38 @@ __global__ __launch_bounds__(kBuildBlock) void hash_insert(const int32_t* __rest
 // Occupied cells of the padded box as one byte per cell (occ zero-filled
 // beforehand; every writer stores the same value, so no atomics are needed),
 // then packed into the bitmap, 32 cells per word.
+// With part set, also the number of valid segments of every workgroup that
+// holds segments, to part[workgroup] (pinned, mapped host memory; the host
+// adds the first ceil(min(segments, cap) / 256) entries).
 __global__ __launch_bounds__(kBuildBlock) void occ_set(const int32_t* __restrict__ coords,
                                                        const uint32_t* __restrict__ valid,
                                                        const uint32_t* __restrict__ seg_count, uint32_t cap,
                                                        int32_t ox, int32_t oy, int32_t oz, uint32_t dx, uint32_t dy,
-                                                       uint8_t* __restrict__ occ) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *seg_count || s >= cap || !valid[s]) return;
-  const uint64_t lin = (uint64_t)(coords[3 * s] - ox) +
-                       (uint64_t)dx * ((uint64_t)(coords[3 * s + 1] - oy) + (uint64_t)dy * (uint64_t)(coords[3 * s + 2] - oz));
-  occ[lin] = 1;
+                                                       uint8_t* __restrict__ occ, uint32_t* part) {
+  __shared__ uint32_t s_c[kBuildBlock / kWave];
+  const uint32_t segments = *seg_count;
+  const uint32_t limit = segments < cap ? segments : cap;
+  const uint32_t s0 = blockIdx.x * kBuildBlock;
+  if (s0 >= limit) return;  // the whole workgroup
+  const uint32_t s = s0 + threadIdx.x;
+  const bool ok = s < limit && valid[s] != 0u;
+  if (ok) {
+    const uint64_t lin = (uint64_t)(coords[3 * s] - ox) +
+                         (uint64_t)dx * ((uint64_t)(coords[3 * s + 1] - oy) +
+                                         (uint64_t)dy * (uint64_t)(coords[3 * s + 2] - oz));
+    occ[lin] = 1;
+  }
+  if (part) {
+    const uint64_t bal = __ballot(ok);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_c[threadIdx.x / kWave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t c = 0;
+#pragma unroll
+      for (int w = 0; w < kBuildBlock / kWave; ++w) c += s_c[w];
+      part[blockIdx.x] = c;
+    }
+  }
 }
 
 // bitmap[w] bit b = occ[32 w + b] (bytes are 0 or 1).  occ holds 32 bytes per
@@ -714,18 +736,30 @@ __global__ __launch_bounds__(kBuildBlock) void cell_ent_set(const int32_t* __res
 
 // ... and the 27-bit block occupancy of every cell whose block lies inside
 // the padded box (zero on the outer layer, where no query block is centred).
+// floor(2^64 / d) + 1: for every 32-bit n, floor(n / d) is the high half of
+// n * magic (the error term n (magic d - 2^64) / 2^64 stays below 1 / d).
+// d >= 2.
+uint64_t div_magic(uint32_t d) {
+  uint64_t f = UINT64_MAX / d;
+  if ((d & (d - 1)) == 0) ++f;  // d divides 2^64
+  return f + 1;
+}
+
 // cell_nbr over the padded box.  Each thread takes four consecutive cells of
-// a row (y, z): per neighbouring row it loads the six occupancy bits of
-// x0-1 .. x0+4 once and forms the four cells' 3-bit windows from them.  Cells
-// on the box's outer layer get 0 (their block would leave the box).
-__global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t xblocks, uint32_t* cell_nbr) {
-  const uint32_t rows = v.bdim[1] * v.bdim[2];
-  const uint32_t quads = (v.bdim[0] + 3) / 4;
-  for (uint32_t blk = blockIdx.x; blk < rows * xblocks; blk += gridDim.x) {
-    const uint32_t row = blk / xblocks;
-    const uint32_t q = (blk - row * xblocks) * kBuildBlock + threadIdx.x;
-    if (q >= quads) continue;
-    const uint32_t y = row % v.bdim[1], z = row / v.bdim[1];
+// a row (y, z), the quads of all rows numbered row by row: per neighbouring
+// row it loads the six occupancy bits of x0-1 .. x0+4 once and forms the four
+// cells' 3-bit windows from them.  Cells on the box's outer layer get 0
+// (their block would leave the box).  The box has at most 2^26 cells, so the
+// quad numbers fit 32 bits; the divisions by the quads per row and by the
+// box's y extent are multiplications by div_magic's reciprocals.
+__global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t quads, uint64_t quads_magic,
+                                                            uint64_t ydim_magic, uint32_t total,
+                                                            uint32_t* cell_nbr) {
+  for (uint32_t t = blockIdx.x * kBuildBlock + threadIdx.x; t < total; t += gridDim.x * kBuildBlock) {
+    const uint32_t row = (uint32_t)__umul64hi((uint64_t)t, quads_magic);
+    const uint32_t q = t - row * quads;
+    const uint32_t z = (uint32_t)__umul64hi((uint64_t)row, ydim_magic);
+    const uint32_t y = row - z * v.bdim[1];
     const uint32_t x0 = 4 * q;
     uint32_t m[4] = {0u, 0u, 0u, 0u};
     if (y >= 1 && y + 2 <= v.bdim[1] && z >= 1 && z + 2 <= v.bdim[2]) {
@@ -741,11 +775,12 @@ __global__ __launch_bounds__(kBuildBlock) void cell_nbr_set(GridView v, uint32_t
         const uint64_t two = (uint64_t)v.bitmap[w] | ((uint64_t)v.bitmap[w + 1] << 32);
         uint32_t win = (uint32_t)(two >> (s & 31));
         if (lin == 0) win <<= 1;
+        // bit0 (x-1), bit1 (x), bit2 (x+1) of the window to bits r, 9 + r and
+        // 18 + r: b3 * 0x10101 places copies of b3 at bits 0, 8 and 16, and
+        // the mask keeps bit 0 of the first, bit 1 of the second and bit 2 of
+        // the third (no copies overlap, so nothing carries).
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const uint32_t b3 = (win >> c) & 7u;  // bit0: x-1, bit1: x, bit2: x+1
-          m[c] |= ((b3 & 1u) << r) | (((b3 >> 1) & 1u) << (9 + r)) | (((b3 >> 2) & 1u) << (18 + r));
-        }
+        for (int c = 0; c < 4; ++c) m[c] |= ((((win >> c) & 7u) * 0x10101u) & 0x40201u) << r;
       }
     }
     uint32_t* out = cell_nbr + (size_t)row * v.bdim[0];
@@ -777,12 +812,6 @@ __global__ __launch_bounds__(kBuildBlock) void valid_bounds(const int32_t* __res
     mx[2] = max(mx[2], k2);
   }
   block_minmax_store(mn, mx, c, part);
-}
-
-__global__ void init_bounds(int32_t* b, uint32_t* extra, int nextra) {
-  if (threadIdx.x < 3) b[threadIdx.x] = INT32_MAX;
-  else if (threadIdx.x < 6) b[threadIdx.x] = INT32_MIN;
-  if ((int)threadIdx.x < nextra) extra[threadIdx.x] = 0;
 }
 
 int grid_for(size_t n, int block, int max_blocks) {
@@ -1010,10 +1039,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   }
   hipError_t e;
   (void)hipEventRecord(G.ev_build0, st);
-  if ((e = G.scalars.reserve(64)) != hipSuccess) return fail(e, "alloc");
-  int32_t* bounds = reinterpret_cast<int32_t*>(G.scalars.ptr) + 8;
-  uint32_t* extra = reinterpret_cast<uint32_t*>(G.scalars.ptr);
-  hipLaunchKernelGGL(init_bounds, dim3(1), dim3(64), 0, st, bounds, extra, 8);
+  if ((e = G.scalars.reserve(64)) != hipSuccess) return fail(e, "alloc");  // [0]: segments (seg_scan_tiles)
   if (!G.host_part &&
       (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_part), (size_t)kMaxBoundBlocks * 8 * sizeof(int32_t),
                          hipHostMallocCoherent | hipHostMallocMapped)) != hipSuccess)
@@ -1127,11 +1153,23 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
                        reinterpret_cast<float*>(G.cov.ptr), reinterpret_cast<float*>(G.axis.ptr),
                        reinterpret_cast<uint32_t*>(G.count.ptr), reinterpret_cast<int32_t*>(G.coord.ptr),
                        reinterpret_cast<uint32_t*>(G.valid.ptr));
-    vb_blocks = grid_for(cap, kBuildBlock, kMaxBoundBlocks);
-    hipLaunchKernelGGL(valid_bounds, dim3(vb_blocks), dim3(kBuildBlock), 0, st,
-                       reinterpret_cast<const int32_t*>(G.coord.ptr), reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                       seg_count, cap, G.host_part);
-    if (!box_from_keys) {
+    if (box_from_keys) {
+      // The valid count comes from occ_set's per-workgroup counts.
+      if (cap_blocks > G.host_valid_cap) {
+        if (G.host_valid) (void)hipHostFree(G.host_valid);
+        G.host_valid = nullptr;
+        G.host_valid_cap = 0;
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&G.host_valid), (size_t)cap_blocks * sizeof(uint32_t),
+                               hipHostMallocCoherent | hipHostMallocMapped)) != hipSuccess)
+          return fail(e, "pinned alloc");
+        G.host_valid_cap = cap_blocks;
+      }
+    } else {
+      // The box of the valid voxels, and their count, before the lookup structures.
+      vb_blocks = grid_for(cap, kBuildBlock, kMaxBoundBlocks);
+      hipLaunchKernelGGL(valid_bounds, dim3(vb_blocks), dim3(kBuildBlock), 0, st,
+                         reinterpret_cast<const int32_t*>(G.coord.ptr),
+                         reinterpret_cast<const uint32_t*>(G.valid.ptr), seg_count, cap, G.host_part);
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "build sync");
       reduce_parts(vb_blocks, vb, &G.num_valid);
     }
@@ -1167,7 +1205,8 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
         hipLaunchKernelGGL(occ_set, dim3(cap_blocks), dim3(kBuildBlock), 0, st,
                            reinterpret_cast<const int32_t*>(G.coord.ptr),
                            reinterpret_cast<const uint32_t*>(G.valid.ptr), seg_count, cap, v.borg[0], v.borg[1],
-                           v.borg[2], dims[0], dims[1], reinterpret_cast<uint8_t*>(G.occ.ptr));
+                           v.borg[2], dims[0], dims[1], reinterpret_cast<uint8_t*>(G.occ.ptr),
+                           box_from_keys ? G.host_valid : nullptr);
         hipLaunchKernelGGL(bitmap_pack, dim3(grid_for(words, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st,
                            reinterpret_cast<const uint8_t*>(G.occ.ptr), words,
                            reinterpret_cast<uint32_t*>(G.bitmap.ptr));
@@ -1181,9 +1220,11 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
                              reinterpret_cast<const uint32_t*>(G.valid.ptr),
                              reinterpret_cast<const float4*>(G.vox.ptr), seg_count, cap, v,
                              reinterpret_cast<float4*>(G.cell_ent.ptr));
-          const uint32_t xblocks = ((dims[0] + 3) / 4 + kBuildBlock - 1) / kBuildBlock;
-          hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for((size_t)xblocks * dims[1] * dims[2], 1, 8192)),
-                             dim3(kBuildBlock), 0, st, v, xblocks, reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
+          const uint32_t quads = (dims[0] + 3) / 4;
+          const uint32_t total = quads * dims[1] * dims[2];  // < 2^32: at most 2^26 cells, rows of >= 5
+          hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for(total, kBuildBlock, 8192)), dim3(kBuildBlock), 0, st, v,
+                             quads, div_magic(quads), div_magic(dims[1]), total,
+                             reinterpret_cast<uint32_t*>(G.cell_nbr.ptr));
           v.cell_nbr = reinterpret_cast<const uint32_t*>(G.cell_nbr.ptr);
           v.cell_ent = reinterpret_cast<const float4*>(G.cell_ent.ptr);
         }
@@ -1216,7 +1257,9 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   if (box_from_keys) {
     segs = G.host_scalars[0];
     if (segs > cap0 && (rc = voxel_stage(segs)) != 0) return rc;
-    reduce_parts(vb_blocks, vb, &G.num_valid);
+    uint64_t c = 0;  // segs <= the last stage's capacity
+    for (uint32_t b = 0; b < (segs + kBuildBlock - 1) / kBuildBlock; ++b) c += G.host_valid[b];
+    G.num_valid = (uint32_t)c;
   }
   G.segments = segs;
   G.n_points = n;
