@@ -55,6 +55,7 @@ struct DeviceGrid {
   uint32_t seg_cap = 0;  // capacity of vox, cov, axis, count, coord and valid, in voxels (grow-only)
   uint32_t num_valid = 0;
   DevBuf vox, cov, axis, count, coord, valid, table, bitmap, occ, cell_nbr, cell_ent;
+  bool occ_zero = false;  // occ holds zeros (bitmap_pack clears what occ_set set)
   GridView view{};
   // host mirrors
   uint32_t* host_scalars = nullptr;  // pinned, 64 words

@@ -706,11 +706,15 @@ __global__ __launch_bounds__(kBuildBlock) void occ_set(const int32_t* __restrict
 
 // bitmap[w] bit b = occ[32 w + b] (bytes are 0 or 1).  occ holds 32 bytes per
 // word, 16-byte aligned.
-__global__ __launch_bounds__(kBuildBlock) void bitmap_pack(const uint8_t* __restrict__ occ, size_t words,
+// The bytes are cleared once read, so occ is all zero again for the next
+// build's occ_set.
+__global__ __launch_bounds__(kBuildBlock) void bitmap_pack(uint8_t* __restrict__ occ, size_t words,
                                                            uint32_t* __restrict__ bitmap) {
   for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < words; w += (size_t)gridDim.x * blockDim.x) {
-    const uint4* q = reinterpret_cast<const uint4*>(occ + 32 * w);
+    uint4* q = reinterpret_cast<uint4*>(occ + 32 * w);
     const uint4 a = q[0], b = q[1];
+    q[0] = make_uint4(0u, 0u, 0u, 0u);
+    q[1] = make_uint4(0u, 0u, 0u, 0u);
     const uint32_t part[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     uint32_t bits = 0;
     // Four 0/1 bytes to four bits: byte k (bit 8k) times 2^(21-7k) lands on bit
@@ -1199,17 +1203,23 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
       if (cells < (1ull << 32)) {
         const size_t words = (size_t)(cells / 32) + 2;
         if ((e = G.bitmap.reserve(words * 4)) != hipSuccess) return fail(e, "alloc bitmap");
+        // The byte map stays all zero between builds (bitmap_pack clears it);
+        // a new allocation, or a build that stopped between occ_set and
+        // bitmap_pack, is filled here.
+        const size_t had = G.occ.bytes;
         if ((e = G.occ.reserve(words * 32)) != hipSuccess) return fail(e, "alloc bitmap");
-        hipLaunchKernelGGL(fill_u32, dim3(grid_for(words * 8, 256, 4096)), dim3(256), 0, st,
-                           reinterpret_cast<uint32_t*>(G.occ.ptr), words * 8, 0u);
+        if (G.occ.bytes != had || !G.occ_zero)
+          hipLaunchKernelGGL(fill_u32, dim3(grid_for(G.occ.bytes / 4, 256, 4096)), dim3(256), 0, st,
+                             reinterpret_cast<uint32_t*>(G.occ.ptr), G.occ.bytes / 4, 0u);
+        G.occ_zero = false;
         hipLaunchKernelGGL(occ_set, dim3(cap_blocks), dim3(kBuildBlock), 0, st,
                            reinterpret_cast<const int32_t*>(G.coord.ptr),
                            reinterpret_cast<const uint32_t*>(G.valid.ptr), seg_count, cap, v.borg[0], v.borg[1],
                            v.borg[2], dims[0], dims[1], reinterpret_cast<uint8_t*>(G.occ.ptr),
                            box_from_keys ? G.host_valid : nullptr);
         hipLaunchKernelGGL(bitmap_pack, dim3(grid_for(words, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st,
-                           reinterpret_cast<const uint8_t*>(G.occ.ptr), words,
-                           reinterpret_cast<uint32_t*>(G.bitmap.ptr));
+                           reinterpret_cast<uint8_t*>(G.occ.ptr), words, reinterpret_cast<uint32_t*>(G.bitmap.ptr));
+        G.occ_zero = true;
         v.bitmap = reinterpret_cast<const uint32_t*>(G.bitmap.ptr);
         if (cell_tables && cells <= kCellTablesMaxCells) {
           const uint32_t nc = (uint32_t)cells;
