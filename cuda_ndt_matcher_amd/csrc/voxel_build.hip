@@ -16,7 +16,8 @@
 //   4. segments       : head flags, tile counts, scan, segment starts.
 //   5. voxel_stats    : one thread per segment: shifted f64 moments, mean,
 //                       covariance, 3x3 Jacobi eigen-regularisation, inverse.
-//   6. cell tables (occupancy word and mean/slot entry per cell of the box),
+//   6. cell tables (occupancy word and mean/slot entry per cell of the box,
+//      the entries written with the occupancy bytes),
 //      or, when the box is too large for them, an open-addressing hash.
 #include <hip/hip_runtime.h>
 
@@ -670,14 +671,17 @@ __global__ __launch_bounds__(kBuildBlock) void hash_insert(const int32_t* __rest
 // Occupied cells of the padded box as one byte per cell (occ zero-filled
 // beforehand; every writer stores the same value, so no atomics are needed),
 // then packed into the bitmap, 32 cells per word.
-// With part set, also the number of valid segments of every workgroup that
-// holds segments, to part[workgroup] (pinned, mapped host memory; the host
-// adds the first ceil(min(segments, cap) / 256) entries).
+// With cell_ent set (the cell tables are built), also each valid voxel's
+// entry {mean, slot} at the same cell index.  With part set, also the number
+// of valid segments of every workgroup that holds segments, to
+// part[workgroup] (pinned, mapped host memory; the host adds the first
+// ceil(min(segments, cap) / 256) entries).
 __global__ __launch_bounds__(kBuildBlock) void occ_set(const int32_t* __restrict__ coords,
                                                        const uint32_t* __restrict__ valid,
                                                        const uint32_t* __restrict__ seg_count, uint32_t cap,
                                                        int32_t ox, int32_t oy, int32_t oz, uint32_t dx, uint32_t dy,
-                                                       uint8_t* __restrict__ occ, uint32_t* part) {
+                                                       uint8_t* __restrict__ occ, const float4* __restrict__ vox,
+                                                       float4* __restrict__ cell_ent, uint32_t* part) {
   __shared__ uint32_t s_c[kBuildBlock / kWave];
   const uint32_t segments = *seg_count;
   const uint32_t limit = segments < cap ? segments : cap;
@@ -690,6 +694,10 @@ __global__ __launch_bounds__(kBuildBlock) void occ_set(const int32_t* __restrict
                          (uint64_t)dx * ((uint64_t)(coords[3 * s + 1] - oy) +
                                          (uint64_t)dy * (uint64_t)(coords[3 * s + 2] - oz));
     occ[lin] = 1;
+    if (cell_ent) {
+      const float4 m = vox[3 * s];
+      cell_ent[lin] = make_float4(m.x, m.y, m.z, __uint_as_float(s));
+    }
   }
   if (part) {
     const uint64_t bal = __ballot(ok);
@@ -724,18 +732,6 @@ __global__ __launch_bounds__(kBuildBlock) void bitmap_pack(uint8_t* __restrict__
       bits |= ((part[j] * 0x00204081u) >> 21 & 0xFu) << (4 * j);
     bitmap[w] = bits;
   }
-}
-
-__global__ __launch_bounds__(kBuildBlock) void cell_ent_set(const int32_t* __restrict__ coords,
-                                                            const uint32_t* __restrict__ valid,
-                                                            const float4* __restrict__ vox,
-                                                            const uint32_t* __restrict__ seg_count, uint32_t cap,
-                                                            GridView v, float4* cell_ent) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= *seg_count || s >= cap || !valid[s]) return;
-  const float4 m = vox[3 * s];
-  cell_ent[box_index(v, coords[3 * s], coords[3 * s + 1], coords[3 * s + 2])] =
-      make_float4(m.x, m.y, m.z, __uint_as_float(s));
 }
 
 // ... and the 27-bit block occupancy of every cell whose block lies inside
@@ -1208,7 +1204,14 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
         // bitmap_pack, is filled here.
         const size_t had = G.occ.bytes;
         if ((e = G.occ.reserve(words * 32)) != hipSuccess) return fail(e, "alloc bitmap");
-        if (G.occ.bytes != had || !G.occ_zero)
+        if (G.occ.bytes != had) G.occ_zero = false;  // a new allocation
+        const bool tables = cell_tables && cells <= kCellTablesMaxCells;
+        if (tables) {
+          const uint32_t nc = (uint32_t)cells;
+          if ((e = G.cell_nbr.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
+          if ((e = G.cell_ent.reserve((size_t)nc * 16)) != hipSuccess) return fail(e, "alloc cell tables");
+        }
+        if (!G.occ_zero)
           hipLaunchKernelGGL(fill_u32, dim3(grid_for(G.occ.bytes / 4, 256, 4096)), dim3(256), 0, st,
                              reinterpret_cast<uint32_t*>(G.occ.ptr), G.occ.bytes / 4, 0u);
         G.occ_zero = false;
@@ -1216,20 +1219,14 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
                            reinterpret_cast<const int32_t*>(G.coord.ptr),
                            reinterpret_cast<const uint32_t*>(G.valid.ptr), seg_count, cap, v.borg[0], v.borg[1],
                            v.borg[2], dims[0], dims[1], reinterpret_cast<uint8_t*>(G.occ.ptr),
+                           reinterpret_cast<const float4*>(G.vox.ptr),
+                           tables ? reinterpret_cast<float4*>(G.cell_ent.ptr) : nullptr,
                            box_from_keys ? G.host_valid : nullptr);
         hipLaunchKernelGGL(bitmap_pack, dim3(grid_for(words, kBuildBlock, 4096)), dim3(kBuildBlock), 0, st,
                            reinterpret_cast<uint8_t*>(G.occ.ptr), words, reinterpret_cast<uint32_t*>(G.bitmap.ptr));
         G.occ_zero = true;
         v.bitmap = reinterpret_cast<const uint32_t*>(G.bitmap.ptr);
-        if (cell_tables && cells <= kCellTablesMaxCells) {
-          const uint32_t nc = (uint32_t)cells;
-          if ((e = G.cell_nbr.reserve((size_t)nc * 4)) != hipSuccess) return fail(e, "alloc cell tables");
-          if ((e = G.cell_ent.reserve((size_t)nc * 16)) != hipSuccess) return fail(e, "alloc cell tables");
-          hipLaunchKernelGGL(cell_ent_set, dim3(cap_blocks), dim3(kBuildBlock), 0, st,
-                             reinterpret_cast<const int32_t*>(G.coord.ptr),
-                             reinterpret_cast<const uint32_t*>(G.valid.ptr),
-                             reinterpret_cast<const float4*>(G.vox.ptr), seg_count, cap, v,
-                             reinterpret_cast<float4*>(G.cell_ent.ptr));
+        if (tables) {
           const uint32_t quads = (dims[0] + 3) / 4;
           const uint32_t total = quads * dims[1] * dims[2];  // < 2^32: at most 2^26 cells, rows of >= 5
           hipLaunchKernelGGL(cell_nbr_set, dim3(grid_for(total, kBuildBlock, 8192)), dim3(kBuildBlock), 0, st, v,
