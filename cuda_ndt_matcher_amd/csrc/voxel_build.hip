@@ -13,7 +13,8 @@
 //                       counts per 4096-key tile), sort_tile_scan (prefix
 //                       over tiles), sort_scatter (wave64 ballot ranking,
 //                       LDS staging, writes in runs).
-//   4. segments       : head flags, tile counts, scan, segment starts.
+//   4. segments       : head counts per tile, then segment starts (each tile
+//                       sums the earlier tiles' counts itself).
 //   5. voxel_stats    : one thread per segment: shifted f64 moments, mean,
 //                       covariance, 3x3 Jacobi eigen-regularisation, inverse.
 //   6. cell tables (occupancy word and mean/slot entry per cell of the box,
@@ -501,13 +502,18 @@ __global__ __launch_bounds__(1024) void seg_scan_tiles(uint32_t* tile_heads, uin
 // wave-striped (item i of lane l in wave w is position tile*4096 + w*1024 +
 // i*64 + l, so (wave, item, lane) order is position order) with every load
 // issued first; a head's segment id is the tile's offset plus the heads of
-// the lower waves, the earlier items of its wave and its lower lanes.
+// the lower waves, the earlier items of its wave and its lower lanes.  The
+// tile's offset is the sum of the earlier tiles' head counts (seg_count's
+// table, a few KB that L2 serves to every workgroup), formed while the keys
+// load; the last tile's workgroup writes the segment count to total and
+// host_total (pinned, mapped host memory).
 template <class K>
 __global__ __launch_bounds__(kBuildBlock) void seg_write(const K* __restrict__ keys, uint32_t n,
-                                                         const uint32_t* __restrict__ tile_offset,
-                                                         uint32_t* __restrict__ seg_start) {
+                                                         const uint32_t* __restrict__ tile_count,
+                                                         uint32_t* __restrict__ seg_start, uint32_t* total,
+                                                         uint32_t* host_total) {
   constexpr int kWaves = kBuildBlock / kWave;
-  __shared__ uint32_t s_wave[kWaves];
+  __shared__ uint32_t s_wave[kWaves], s_pre[kWaves];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const uint32_t first = blockIdx.x * kSortTile + wave * (kSortItems * kWave) + lane;
   K cur[kSortItems], prev[kSortItems];
@@ -518,6 +524,10 @@ __global__ __launch_bounds__(kBuildBlock) void seg_write(const K* __restrict__ k
     cur[i] = keys[c];
     prev[i] = keys[c > 0 ? c - 1 : 0];
   }
+  uint32_t pre = 0;
+  for (uint32_t t = tid; t < blockIdx.x; t += kBuildBlock) pre += tile_count[t];
+#pragma unroll
+  for (int off = kWave / 2; off >= 1; off >>= 1) pre += __shfl_xor(pre, off, kWave);
   uint64_t heads[kSortItems];
   uint32_t count = 0;
 #pragma unroll
@@ -526,9 +536,21 @@ __global__ __launch_bounds__(kBuildBlock) void seg_write(const K* __restrict__ k
     heads[i] = __ballot(idx < n && (idx == 0 || cur[i] != prev[i]));
     count += (uint32_t)__popcll(heads[i]);
   }
-  if (lane == 0) s_wave[wave] = count;
+  if (lane == 0) {
+    s_wave[wave] = count;
+    s_pre[wave] = pre;
+  }
   __syncthreads();
-  uint32_t seg = tile_offset[blockIdx.x];
+  uint32_t seg = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) seg += s_pre[w];
+  if (blockIdx.x + 1 == gridDim.x && tid == 0) {
+    uint32_t all = seg;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) all += s_wave[w];
+    *total = all;
+    *host_total = all;
+  }
 #pragma unroll
   for (int w = 0; w < kWaves; ++w) seg += w < wave ? s_wave[w] : 0u;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -867,9 +889,8 @@ hipError_t run_segments(DeviceGrid& G, size_t n, hipStream_t st, uint32_t* segme
   uint32_t* tile_heads = reinterpret_cast<uint32_t*>(G.tile_heads.ptr);
   uint32_t* scalars = reinterpret_cast<uint32_t*>(G.scalars.ptr);
   hipLaunchKernelGGL(seg_count<K>, dim3(tiles), dim3(kBuildBlock), 0, st, keys, (uint32_t)n, tile_heads);
-  hipLaunchKernelGGL(seg_scan_tiles, dim3(1), dim3(1024), 0, st, tile_heads, tiles, scalars, G.host_scalars);
   hipLaunchKernelGGL(seg_write<K>, dim3(tiles), dim3(kBuildBlock), 0, st, keys, (uint32_t)n, tile_heads,
-                     reinterpret_cast<uint32_t*>(G.seg_start.ptr));
+                     reinterpret_cast<uint32_t*>(G.seg_start.ptr), scalars, G.host_scalars);
   if (!segments_out) return hipGetLastError();  // read from host_scalars after a later synchronisation
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) return e;
@@ -1039,7 +1060,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   }
   hipError_t e;
   (void)hipEventRecord(G.ev_build0, st);
-  if ((e = G.scalars.reserve(64)) != hipSuccess) return fail(e, "alloc");  // [0]: segments (seg_scan_tiles)
+  if ((e = G.scalars.reserve(64)) != hipSuccess) return fail(e, "alloc");  // [0]: segments (seg_write)
   if (!G.host_part &&
       (e = hipHostMalloc(reinterpret_cast<void**>(&G.host_part), (size_t)kMaxBoundBlocks * 8 * sizeof(int32_t),
                          hipHostMallocCoherent | hipHostMallocMapped)) != hipSuccess)
@@ -1127,7 +1148,7 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   e = G.key64 ? run_segments<uint64_t>(G, n, st, box_from_keys ? nullptr : &segs)
               : run_segments<uint32_t>(G, n, st, box_from_keys ? nullptr : &segs);
   if (e != hipSuccess) return fail(e, "segments");
-  const uint32_t* seg_count = reinterpret_cast<const uint32_t*>(G.scalars.ptr);  // written by seg_scan_tiles
+  const uint32_t* seg_count = reinterpret_cast<const uint32_t*>(G.scalars.ptr);  // written by seg_write
   int32_t vb[6];
   int vb_blocks = 0;
   // Queues voxel_stats, the valid count and the lookup structures for up to
