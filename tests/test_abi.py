@@ -122,3 +122,37 @@ def test_product_does_not_import_oracle():
         for f in files:
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
                 assert not pattern.search(open(os.path.join(dirpath, f)).read()), f
+
+
+def test_result_conversions_read_every_field():
+    """The one-result path (field by field) and the batch path (one
+    structured numpy view) turn ndt_result structs into identical
+    AlignResults, field for field, with every field of the struct set to a
+    distinct value."""
+    from cuda_ndt_matcher_amd import _lib, matcher
+
+    rng = np.random.default_rng(3)
+    arr = (_lib.NdtResult * 9)()
+    for i in range(9):
+        r = arr[i]
+        for k in range(6):
+            r.pose[k] = rng.normal()
+        for k in range(36):
+            r.hessian[k] = rng.normal()
+        r.status, r.converged = i % 4, i % 2
+        r.score, r.transform_probability, r.nvtl = rng.normal(size=3)
+        r.iterations, r.num_correspondences = 3 + i, 1000 + i
+        r.oscillation_count, r.pair_evaluations = i, (1 << 31) + i
+    batch = matcher._results(arr, 7)
+    assert len(batch) == 7
+    for i, b in enumerate(batch):
+        a = matcher._result(arr[i])
+        r = arr[i]
+        assert np.array_equal(a.pose_vector, list(r.pose)) and np.array_equal(b.pose_vector, a.pose_vector)
+        assert np.array_equal(a.hessian, np.reshape(list(r.hessian), (6, 6)))
+        assert np.array_equal(b.hessian, a.hessian)
+        for f in ("converged", "status", "score", "transform_probability", "nvtl", "iterations",
+                  "num_correspondences", "oscillation_count", "pair_evaluations"):
+            assert getattr(a, f) == getattr(b, f) and type(getattr(a, f)) is type(getattr(b, f)), f
+        assert a.status == matcher.STATUS_NAMES[i % 4] and a.converged == bool(i % 2)
+        assert a.score == r.score and a.iterations == 3 + i and a.pair_evaluations == (1 << 31) + i
