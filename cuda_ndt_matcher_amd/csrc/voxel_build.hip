@@ -173,13 +173,24 @@ __device__ inline int32_t key_of(float p, const KeyParams& kp) {
 // block also counts the first pass's digits of its tile:
 // tile_hist[d * tstride + t].  Every load is issued before any key is formed;
 // indices past n are clamped to n - 1 and their items not counted.
+// Workgroups are handed to the eight XCDs in turn (workgroup b to XCD b % 8).
+// xcd_tile gives the workgroups of one XCD consecutive tiles, so that the
+// neighbouring entries of a tile-count row (a 64-byte line holds 16 tiles)
+// are written through one L2 instead of eight.  A bijection on [0, tiles):
+// XCD x holds tiles / 8 workgroups, plus one when x < tiles % 8, and takes
+// the tiles from x * (tiles / 8) + min(x, tiles % 8) on.
+__device__ inline uint32_t xcd_tile(uint32_t b, uint32_t tiles) {
+  const uint32_t x = b & 7u, i = b >> 3, q = tiles >> 3, r = tiles & 7u;
+  return x * q + (x < r ? x : r) + i;
+}
+
 template <class K>
 __global__ __launch_bounds__(kBuildBlock) void keygen_count(const float* __restrict__ pts, uint32_t n, KeyParams kp,
                                                             K* __restrict__ keys, uint32_t* __restrict__ tile_hist,
                                                             uint32_t tstride) {
   __shared__ uint32_t s_hist[kRadix];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t first = tile * kSortTile + wave * (kSortItems * kWave) + lane;
   float v[kSortItems][3];
 #pragma unroll
@@ -246,7 +257,7 @@ __global__ __launch_bounds__(kBuildBlock) void sort_count(const K* __restrict__ 
                                                           uint32_t* __restrict__ tile_hist, uint32_t tstride) {
   __shared__ uint32_t s_hist[kRadix];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t first = tile * kSortTile + wave * (kSortItems * kWave) + lane;
   K key[kSortItems];
 #pragma unroll
@@ -348,7 +359,7 @@ __global__ __launch_bounds__(kBuildBlock) void sort_scatter(const K* __restrict_
   __shared__ uint32_t s_vals[kSortTile];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-  const uint32_t tile = blockIdx.x;
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t base = tile * kSortTile;
   const uint32_t first = base + wave * (kSortItems * kWave) + lane;
 
@@ -449,8 +460,7 @@ __global__ __launch_bounds__(kBuildBlock) void sort_scatter(const K* __restrict_
 }
 
 // Segment heads: flag[i] = (i == 0 || key[i] != key[i-1]).  Three phases:
-// per-tile head counts, a single-workgroup scan over tiles, and the write of
-// segment starts.
+// per-tile head counts, then the write of segment starts.
 template <class K>
 __global__ __launch_bounds__(kBuildBlock) void seg_count(const K* __restrict__ keys, uint32_t n,
                                                          uint32_t* __restrict__ tile_heads) {
