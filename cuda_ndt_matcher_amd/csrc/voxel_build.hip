@@ -466,10 +466,21 @@ __global__ __launch_bounds__(kBuildBlock) void seg_count(const K* __restrict__ k
                                                          uint32_t* __restrict__ tile_heads) {
   __shared__ uint32_t s[kBuildBlock / kWave];
   const uint32_t base = blockIdx.x * kSortTile;
+  // Every load issued before any comparison (positions past n re-read the
+  // last key and are not counted; n >= 1).
+  K cur[kSortItems], prev[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = base + threadIdx.x + i * kBuildBlock;
+    const uint32_t q = idx < n ? idx : n - 1;
+    cur[i] = keys[q];
+    prev[i] = keys[q > 0 ? q - 1 : 0];
+  }
   uint32_t c = 0;
-  for (uint32_t j = threadIdx.x; j < kSortTile; j += kBuildBlock) {
-    const uint32_t i = base + j;
-    if (i < n && (i == 0 || keys[i] != keys[i - 1])) ++c;
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t idx = base + threadIdx.x + i * kBuildBlock;
+    c += (idx < n && (idx == 0 || cur[i] != prev[i])) ? 1u : 0u;
   }
   for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, kWave);
   if ((threadIdx.x & (kWave - 1)) == 0) s[threadIdx.x / kWave] = c;
