@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -154,7 +155,8 @@ __global__ __launch_bounds__(kBuildBlock) void key_bounds(const float* __restric
 struct KeyParams {
   float res;
   float inv;  // 1 / res in f32 (mul keys)
-  int mul;    // 0: floor(p / res) (map voxels); 1: floor(p * inv) (downsampling)
+  int mul;    // 0: floor(p / res) (map voxels); 1: floor(p * inv) (downsampling, and map
+              // voxels at a power-of-two res, where the two agree)
   int32_t kmin[3];
   int bits[3];
   int passes;
@@ -1107,15 +1109,21 @@ int build_grid(DeviceGrid& G, size_t n, float res, uint32_t min_points, bool cel
   const int kb_blocks = grid_for(n, kBuildBlock, kMaxBoundBlocks);
   // The per-workgroup partials go straight to pinned host memory (coherent and
   // mapped), so the stream carries no copy ahead of the synchronisation.
-  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, 1.0f / res, 0,
+  // For a power-of-two resolution, p * (1 / res) is the same correctly rounded
+  // value as p / res for every p (both round the one real value p * 2^-k), so
+  // the keys take the multiplication instead of the division.
+  int res_exp = 0;
+  const float res_inv = 1.0f / res;
+  const int key_mul = (std::frexp(res, &res_exp) == 0.5f && res_inv * res == 1.0f) ? 1 : 0;
+  hipLaunchKernelGGL(key_bounds, dim3(kb_blocks), dim3(kBuildBlock), 0, st, G.points, n, res, res_inv, key_mul,
                      (reinterpret_cast<uintptr_t>(G.points) & 15u) == 0 ? 1 : 0, G.host_part);
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e, "bounds sync");
   int32_t kb[6];
   reduce_parts(kb_blocks, kb, nullptr);
   KeyParams kp;
   kp.res = res;
-  kp.inv = 1.0f / res;
-  kp.mul = 0;
+  kp.inv = res_inv;
+  kp.mul = key_mul;
   int total_bits = 0;
   for (int a = 0; a < 3; ++a) {
     const int64_t span = (int64_t)kb[3 + a] - (int64_t)kb[a];
